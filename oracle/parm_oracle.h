@@ -1,0 +1,142 @@
+/*
+ * parm_oracle.h -- CPU oracle for the ParM per-timestep MD hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (wackywendell/parm, double precision, VEC2D / VEC3D builds) used
+ * as the checker by tests/, by __graft_entry__.smoke() and by the CPU-baseline
+ * leg of bench.py.  The product (libparm_hip.so and the parm_amd package)
+ * never links, loads or calls it.
+ *
+ * Every function restates a reference function; the citation is on each
+ * declaration (paths relative to the reference repository root).
+ *
+ * Conventions shared with the reference:
+ *   - positions are never wrapped; periodicity lives only in the minimum
+ *     image  diff(r1, r2) = remainder(r1 - r2, L)  per component
+ *     (src/box.hpp:68-78, 97-103);
+ *   - |r|^2 is summed as (x*x + y*y) + z*z, the order Eigen's vectorised
+ *     redux uses for a 3-vector of doubles (src/trackers.cpp:27,66 call
+ *     Vec::norm()).  This order is an assumption about the compiled
+ *     reference: the scalar Eigen unroller would give x*x + (y*y + z*z).
+ *   - pair lists are in reference order: for member i ascending, for member
+ *     j < i ascending, pair (first = i, last = j)  (src/trackers.cpp:56-69).
+ *
+ * Pair interaction kinds (per-type parameter tables; a "type" is one distinct
+ * per-atom parameter tuple):
+ *   OR_LJ_ATTRACT_REPULSE  NListed<IEpsSigCutAtom, LJAttractRepulsePair>
+ *       eps_tab : ntypes x ntypes epsilon rows (row = type of atom1)
+ *       sig_tab : sigma per type;  p3_tab : sigcut per type
+ *   OR_REPULSION           NListed<EpsSigExpAtom, RepulsionPair>
+ *       eps_tab : eps per type; sig_tab : sigma per type; p3_tab : exponent
+ */
+#ifndef PARM_ORACLE_H
+#define PARM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_LJ_ATTRACT_REPULSE = 1, OR_REPULSION = 2 };
+
+/* remainder(dx, L): OriginBox::diff per component (src/box.hpp:68-78). */
+double or_vec_mod1(double dx, double L);
+
+/* One pair, atom1 of type t1 at r1, atom2 of type t2 at r2 in an OriginBox.
+ * Writes the force on atom1 (ndim doubles) and returns the pair energy.
+ * LJ: src/interaction.hpp:1251-1299; Repulsion: src/interaction.hpp:1528-1566. */
+double or_pair(int ndim, int kind, int ntypes, const double *eps_tab,
+               const double *sig_tab, const double *p3_tab, int t1, int t2,
+               const double *r1, const double *r2, const double *L,
+               double *f_out);
+
+/* NeighborList::update_list(true) pair predicate over members, O(M^2) brute
+ * force exactly as src/trackers.cpp:56-69.  member[k] = atom index of the k-th
+ * added member; diam[k] = its diameter.  Writes up to cap pairs (first, last)
+ * as atom indices; returns the pair count (may exceed cap: then call again). */
+int64_t or_nlist_brute(int ndim, int64_t nmembers, const int32_t *member,
+                       const double *diam, const double *x, const double *L,
+                       double skin, int64_t cap, int32_t *first,
+                       int32_t *last);
+
+/* Same pair set and order as or_nlist_brute, in O(M) with a periodic cell
+ * grid and the identical predicate.  Used to check the device list at sizes
+ * where the O(M^2) loop is too slow. */
+int64_t or_nlist_cells(int ndim, int64_t nmembers, const int32_t *member,
+                       const double *diam, const double *x, const double *L,
+                       double skin, int64_t cap, int32_t *first,
+                       int32_t *last);
+
+/* The O(N) skin test of NeighborList::update_list(false)
+ * (src/trackers.cpp:23-53): running top-2 of |x_i - lastloc_i| over members
+ * in order, returns 1 iff the sum of the two largest reaches skin. */
+int or_skin_check(int ndim, int64_t nmembers, const int32_t *member,
+                  const double *x, const double *lastlocs, double skin);
+
+/* NListed<A,P>::set_forces / energy / pressure / stress over a pair list in
+ * list order (src/interaction.hpp:2153-2291), Newton-3 scatter
+ *   f[first] += f_pair ; f[last] -= f_pair.
+ * f (natoms x ndim) is ACCUMULATED into (the caller zeroes it, as
+ * AtomGroup::reset_forces does).  fabs_sum (natoms, may be NULL) accumulates
+ * sum_j |f_ij| per atom, the normaliser for force comparisons.  U, W
+ * (sum r.f) and S (ndim x ndim, sum r f^T; may be NULL) are written.
+ * Also counts contacts (E != 0) and overlaps (E > 0)
+ * (src/interaction.hpp:2125-2151). */
+void or_forces(int ndim, int64_t natoms, const double *x, const double *L,
+               int kind, int ntypes, const double *eps_tab,
+               const double *sig_tab, const double *p3_tab, const int32_t *type,
+               int64_t npairs, const int32_t *first, const int32_t *last,
+               double *f, double *fabs_sum, double *U, double *W, double *S,
+               uint64_t *contacts, uint64_t *overlaps);
+
+/* ---------------- whole-system restatement (AoS, reference layout) -------- */
+
+/* Opaque oracle system: struct Atom {Vec x,v,a,f; flt m;} per atom
+ * (src/box.hpp:234-249), OriginBox, one NeighborList, one NListed<A,P>. */
+typedef struct or_system or_system;
+
+or_system *or_sys_create(int ndim, int64_t natoms, const double *masses,
+                         const double *L, int kind, int ntypes,
+                         const double *eps_tab, const double *sig_tab,
+                         const double *p3_tab, const int32_t *type,
+                         int64_t nmembers, const int32_t *member,
+                         const double *diam, double skin, int brute_rebuild);
+void or_sys_destroy(or_system *s);
+/* Copy state in/out as natoms x ndim row-major arrays (any pointer may be NULL). */
+void or_sys_set_state(or_system *s, const double *x, const double *v,
+                      const double *a, const double *f);
+void or_sys_get_state(const or_system *s, double *x, double *v, double *a,
+                      double *f);
+/* NeighborList::update_list(force)  (src/trackers.cpp:19-85); returns 1 if
+ * the list was rebuilt. */
+int or_sys_update_list(or_system *s, int force);
+uint32_t or_sys_which(const or_system *s);
+int64_t or_sys_numpairs(const or_system *s);
+int64_t or_sys_get_pairs(const or_system *s, int64_t cap, int32_t *first,
+                         int32_t *last);
+/* Collection::set_forces(constraints_and_a) (src/collection.cpp:159-179)
+ * with the single NListed interaction. */
+void or_sys_set_forces(or_system *s, int and_a);
+/* Observables (src/collection.cpp:73-142, src/box.cpp:239-260, 401-411). */
+double or_sys_potential_energy(or_system *s);
+double or_sys_virial(or_system *s);
+void or_sys_stress(or_system *s, double *S);
+double or_sys_kinetic_energy(const or_system *s, const double *originv);
+void or_sys_momentum(const or_system *s, double *p);
+double or_sys_mass(const or_system *s);
+double or_sys_temp(const or_system *s, int minuscomv);
+/* Collection::initialize() for a collection whose only tracker is the list
+ * (src/collection.cpp:13-19). */
+void or_sys_initialize(or_system *s);
+/* CollectionVerlet::timestep (src/collection.cpp:442-469), nsteps times. */
+void or_sys_verlet(or_system *s, double dt, int64_t nsteps);
+/* CollectionDamped::set_constants + timestep (src/collection.cpp:342-381). */
+void or_damped_constants(double dt, double damping, double *c0, double *c1,
+                         double *c2);
+void or_sys_damped(or_system *s, double dt, double damping, int64_t nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
