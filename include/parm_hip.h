@@ -1,0 +1,197 @@
+/*
+ * parm_hip.h -- C ABI of libparm_hip.so: the MI355X (gfx950) device path for
+ * ParM's per-timestep MD hot path.
+ *
+ * One `parm_system` is one ParM AtomVec (src/box.hpp:441-479) with its
+ * OriginBox (src/box.hpp:97-157), one NeighborList (src/trackers.hpp:157-214)
+ * and one neighbour-listed pair interaction, either
+ *   NListed<IEpsSigCutAtom, LJAttractRepulsePair>  (SWIG name LJAttractRepulse)
+ *   NListed<EpsSigExpAtom,  RepulsionPair>         (SWIG name Repulsion)
+ * (src/interaction.hpp:1876-1945, 989-1018, 1251-1299, 1454-1466, 1528-1590;
+ * src/sim.i:629,635), driven by CollectionVerlet / CollectionDamped
+ * (src/collection.hpp:273-295, 360-374).  All state lives on the current HIP
+ * device at create time; every call is ordered on the system's own stream.
+ *
+ * Conventions
+ *   - Arrays of vectors are natoms x ndim, row-major, double, in the caller's
+ *     atom order (the AtomVec index).  The device keeps its own spatially
+ *     sorted order internally; it is never visible through this ABI.
+ *   - Every function returns 0 on success or a negative parm_status.  The
+ *     message of the last failure on the calling thread is parm_last_error().
+ *     Nothing throws across the ABI.
+ *   - Precision PARM_FP64 computes everything in double; PARM_MIXED evaluates
+ *     pair distances and forces in float, keeping positions, velocities, the
+ *     integrator update and the energy/virial sums in double.
+ *   - No function falls back to a CPU path: without a usable gfx950 device
+ *     they fail with PARM_E_DEVICE.
+ */
+#ifndef PARM_HIP_H
+#define PARM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct parm_system parm_system;
+
+enum parm_status {
+    PARM_OK = 0,
+    PARM_E_INVALID = -1,   /* bad argument (maps to std::invalid_argument) */
+    PARM_E_DEVICE = -2,    /* HIP runtime / device failure, or no GPU      */
+    PARM_E_NOMEM = -3,     /* device allocation failed                    */
+    PARM_E_UNSUPPORTED = -4, /* configuration the device path lacks        */
+    PARM_E_NONFINITE = -5, /* a position became NaN or Inf                */
+    PARM_E_STATE = -6      /* call out of order (e.g. no interaction set)  */
+};
+
+enum parm_precision { PARM_FP64 = 0, PARM_MIXED = 1 };
+
+enum parm_pair_kind {
+    PARM_PAIR_NONE = 0,
+    PARM_PAIR_LJ_ATTRACT_REPULSE = 1, /* NListed<IEpsSigCutAtom, LJAttractRepulsePair> */
+    PARM_PAIR_REPULSION = 2           /* NListed<EpsSigExpAtom, RepulsionPair>        */
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *parm_last_error(void);
+
+/* ABI version: major changes break callers. */
+int parm_version(int *major, int *minor);
+
+/* Number of visible HIP devices (0 without a GPU; never an error). */
+int parm_device_count(int *count);
+
+/* ---------------------------------------------------------------- system */
+
+/* AtomVec(vector<double> masses) / AtomVec(N, mass) (src/box.hpp:455-464):
+ * natoms atoms with the given masses, x = v = a = f = 0. ndim is 2 or 3
+ * (the reference's VEC2D / VEC3D builds, src/vecrand.hpp:3-16). */
+int parm_system_create(int ndim, int precision, int64_t natoms,
+                       const double *masses, parm_system **out);
+int parm_system_destroy(parm_system *sys);
+int parm_system_info(const parm_system *sys, int *ndim, int *precision,
+                     int64_t *natoms);
+
+/* OriginBox(Vec size) (src/box.hpp:102): edge lengths L[ndim] > 0. */
+int parm_set_box(parm_system *sys, const double *L);
+
+/* Atom.x / .v / .a / .f assignment (src/box.hpp:234-249): any pointer may be
+ * NULL to leave that field.  Uploads are ordered before later work. */
+int parm_set_state(parm_system *sys, const double *x, const double *v,
+                   const double *a, const double *f);
+/* Read Atom.x / .v / .a / .f back (blocks until the device is done). */
+int parm_get_state(parm_system *sys, double *x, double *v, double *a,
+                   double *f);
+/* Atom.m assignment. */
+int parm_set_masses(parm_system *sys, const double *masses);
+
+/* ----------------------------------------------------------- NeighborList */
+
+/* NeighborList(box, atoms, skin) followed by add(atom_k, diameter_k) for
+ * k = 0..nmembers-1 in order (src/trackers.cpp:10-17, trackers.hpp:194-201;
+ * NListed::add passes A::max_size() as the diameter, interaction.hpp:1906).
+ * member[k] is the atom index of the k-th added atom; duplicates are
+ * rejected with PARM_E_INVALID like SubGroup::add (box.hpp:497-503).
+ * Like add(), this marks the list for a rebuild at the next update. */
+int parm_nlist_set(parm_system *sys, double skin, int64_t nmembers,
+                   const int32_t *member, const double *diameter);
+
+/* NeighborList::update_list(force) (src/trackers.cpp:19-85): the O(N) skin
+ * test (two largest displacements since the last rebuild sum to >= skin),
+ * then, if needed or forced, a rebuild.  *rebuilt (may be NULL) is 1 if the
+ * list was rebuilt. */
+int parm_nlist_update(parm_system *sys, int force, int *rebuilt);
+
+/* NeighborList::which() (rebuild counter) and numpairs() (trackers.hpp:187-188). */
+int parm_nlist_which(parm_system *sys, uint32_t *which);
+int parm_nlist_numpairs(parm_system *sys, int64_t *npairs);
+
+/* NeighborList::get(i) / begin()..end() (trackers.hpp:205-212): the pair list
+ * in the reference's order (member rank of `first` ascending, then of `last`
+ * ascending, rank(last) < rank(first)) as atom indices.  Writes
+ * min(cap, numpairs) pairs; *npairs receives numpairs. */
+int parm_nlist_pairs(parm_system *sys, int64_t cap, int32_t *first,
+                     int32_t *last, int64_t *npairs);
+
+/* ------------------------------------------------------ pair interaction */
+
+/* NListed<IEpsSigCutAtom, LJAttractRepulsePair> per-atom parameters, as a
+ * type table: atom i has type atom_type[i] in [0, ntypes); type t carries
+ * epsilons row eps_rows[t*ntypes .. +ntypes), indx t, sigma[t], sigcut[t]
+ * (IEpsSigCutAtom(a, epsilons, indx, sigma, cut), interaction.hpp:997-1003).
+ * Pair parameters follow LJAttractRepulsePair's constructor
+ * (interaction.hpp:1256-1269).  ntypes <= 64. */
+int parm_pair_lj_attract_repulse(parm_system *sys, int ntypes,
+                                 const double *eps_rows, const double *sigma,
+                                 const double *sigcut,
+                                 const int32_t *atom_type);
+
+/* NListed<EpsSigExpAtom, RepulsionPair>: type t carries eps[t], sigma[t],
+ * exponent[t] (EpsSigExpAtom(a, eps, sigma, exponent), interaction.hpp:1457).
+ * ntypes <= 64. */
+int parm_pair_repulsion(parm_system *sys, int ntypes, const double *eps,
+                        const double *sigma, const double *exponent,
+                        const int32_t *atom_type);
+
+/* ------------------------------------------------------ forces, sums */
+
+/* Collection::set_forces(and_a) with this one interaction
+ * (src/collection.cpp:159-179): f = sum of pair forces over the current list;
+ * if and_a, a = f/m (a = 0 for m <= 0 or m = inf). */
+int parm_set_forces(parm_system *sys, int and_a);
+
+/* One fused pass over the current list at the current positions:
+ *   *U        NListed::energy            (interaction.hpp:2153-2163)
+ *   *W        NListed::pressure (sum r.f) (interaction.hpp:2247-2261)
+ *   S[ndim*ndim] NListed::stress (sum r f^T, row-major) (:2277-2291)
+ *   *contacts, *overlaps  NListed::contacts / overlaps   (:2125-2151)
+ * Any output may be NULL.  Forces are not modified. */
+int parm_pair_sums(parm_system *sys, double *U, double *W, double *S,
+                   uint64_t *contacts, uint64_t *overlaps);
+
+/* AtomGroup::kinetic_energy(originv) (box.cpp:401-411), momentum()
+ * (box.cpp:251-260) and mass() (box.cpp:239-247).  originv may be NULL (0). */
+int parm_kinetic_energy(parm_system *sys, const double *originv, double *K);
+int parm_momentum(parm_system *sys, double *p, double *mass);
+
+/* Largest and mean |f_i| over all atoms (relaxation criteria). */
+int parm_force_stats(parm_system *sys, double *fmax, double *fmean);
+
+/* ------------------------------------------------------------ integrators */
+
+/* Collection::initialize() for a collection whose trackers are exactly this
+ * system's NeighborList and whose interactions are this system's pair
+ * (src/collection.cpp:13-19): update_list(false); set_forces(true);
+ * update_list(false). */
+int parm_initialize(parm_system *sys);
+
+/* nsteps x CollectionVerlet::timestep() (src/collection.cpp:442-469),
+ * including the NeighborList update at the end of each step. */
+int parm_verlet_steps(parm_system *sys, double dt, int64_t nsteps);
+
+/* nsteps x CollectionDamped::timestep() (src/collection.cpp:356-381) with the
+ * constants of CollectionDamped::set_constants (:342-354). dt > 0. */
+int parm_damped_steps(parm_system *sys, double dt, double damping,
+                      int64_t nsteps);
+
+/* Wait for all work queued on the system. */
+int parm_synchronize(parm_system *sys);
+
+/* ------------------------------------------------------ instrumentation */
+/* No reference counterpart (the reference has no timers); for benchmarks.
+ * While enabled, every fused step-kernel launch (force + kick + next drift +
+ * skin test) and every list rebuild issued by parm_verlet_steps /
+ * parm_damped_steps is bracketed by HIP events on the system's stream.
+ * parm_profile_read waits for the stream and returns the summed durations in
+ * milliseconds and the counts since the last reset (reset != 0 clears them). */
+int parm_profile_enable(parm_system *sys, int enable);
+int parm_profile_read(parm_system *sys, int reset, double *step_ms,
+                      int64_t *step_launches, double *rebuild_ms,
+                      int64_t *rebuilds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARM_HIP_H */

@@ -1,0 +1,278 @@
+"""ctypes binding of libparm_hip.so (the C ABI in include/parm_hip.h).
+
+Loads the in-tree library built by ``__graft_entry__.build()``.  There is no
+fallback: if the library is missing, or no gfx950 device is present, the
+calls raise.  Errors map to the exceptions the reference's SWIG layer raises
+for the same C++ exceptions (src/sim.i:139-157): invalid arguments ->
+ValueError, everything else -> RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libparm_hip.so")
+
+OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_NONFINITE, E_STATE = 0, -1, -2, -3, -4, -5, -6
+FP64, MIXED = 0, 1
+PAIR_LJ_ATTRACT_REPULSE, PAIR_REPULSION = 1, 2
+
+#: every function include/parm_hip.h declares (checked by the CPU tests)
+EXPORTS = (
+    "parm_last_error", "parm_version", "parm_device_count",
+    "parm_system_create", "parm_system_destroy", "parm_system_info",
+    "parm_set_box", "parm_set_state", "parm_get_state", "parm_set_masses",
+    "parm_nlist_set", "parm_nlist_update", "parm_nlist_which",
+    "parm_nlist_numpairs", "parm_nlist_pairs",
+    "parm_pair_lj_attract_repulse", "parm_pair_repulsion",
+    "parm_set_forces", "parm_pair_sums", "parm_kinetic_energy", "parm_momentum",
+    "parm_force_stats", "parm_initialize", "parm_verlet_steps",
+    "parm_damped_steps", "parm_synchronize", "parm_profile_enable",
+    "parm_profile_read",
+)
+
+
+class ParmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (parm status {code})")
+        self.code = code
+
+
+class ParmValueError(ValueError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (parm status {code})")
+        self.code = code
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+
+def lib():
+    """The loaded library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(the device path has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        i64, i32, dbl = C.c_int64, C.c_int, C.c_double
+        sig = {
+            "parm_last_error": (C.c_char_p, []),
+            "parm_version": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "parm_device_count": (i32, [C.POINTER(C.c_int)]),
+            "parm_system_create": (i32, [i32, i32, i64, _dp, C.POINTER(_vp)]),
+            "parm_system_destroy": (i32, [_vp]),
+            "parm_system_info": (i32, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int64)]),
+            "parm_set_box": (i32, [_vp, _dp]),
+            "parm_set_state": (i32, [_vp, _dp, _dp, _dp, _dp]),
+            "parm_get_state": (i32, [_vp, _dp, _dp, _dp, _dp]),
+            "parm_set_masses": (i32, [_vp, _dp]),
+            "parm_nlist_set": (i32, [_vp, dbl, i64, _ip, _dp]),
+            "parm_nlist_update": (i32, [_vp, i32, C.POINTER(C.c_int)]),
+            "parm_nlist_which": (i32, [_vp, C.POINTER(C.c_uint32)]),
+            "parm_nlist_numpairs": (i32, [_vp, C.POINTER(C.c_int64)]),
+            "parm_nlist_pairs": (i32, [_vp, i64, _ip, _ip, C.POINTER(C.c_int64)]),
+            "parm_pair_lj_attract_repulse": (i32, [_vp, i32, _dp, _dp, _dp, _ip]),
+            "parm_pair_repulsion": (i32, [_vp, i32, _dp, _dp, _dp, _ip]),
+            "parm_set_forces": (i32, [_vp, i32]),
+            "parm_pair_sums": (i32, [_vp, _dp, _dp, _dp, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint64)]),
+            "parm_kinetic_energy": (i32, [_vp, _dp, _dp]),
+            "parm_momentum": (i32, [_vp, _dp, _dp]),
+            "parm_force_stats": (i32, [_vp, _dp, _dp]),
+            "parm_initialize": (i32, [_vp]),
+            "parm_verlet_steps": (i32, [_vp, dbl, i64]),
+            "parm_damped_steps": (i32, [_vp, dbl, dbl, i64]),
+            "parm_synchronize": (i32, [_vp]),
+            "parm_profile_enable": (i32, [_vp, i32]),
+            "parm_profile_read": (i32, [_vp, i32, _dp, C.POINTER(C.c_int64), _dp,
+                                        C.POINTER(C.c_int64)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        msg = lib().parm_last_error().decode(errors="replace")
+        if rc == E_INVALID:
+            raise ParmValueError(rc, msg)
+        raise ParmError(rc, msg)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().parm_device_count(C.byref(n)))
+    return n.value
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+class System:
+    """One device system: AtomVec + OriginBox + NeighborList + one NListed
+    pair interaction (see include/parm_hip.h)."""
+
+    def __init__(self, ndim, masses, precision=FP64):
+        masses = _f64(masses).ravel()
+        self.ndim = int(ndim)
+        self.n = masses.size
+        self.precision = precision
+        h = _vp()
+        _check(lib().parm_system_create(self.ndim, int(precision), self.n, _d(masses),
+                                        C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().parm_system_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    # -- state
+    def set_box(self, L):
+        L = _f64(np.broadcast_to(np.asarray(L, float), (self.ndim,)))
+        _check(lib().parm_set_box(self.h, _d(L)))
+
+    def set_state(self, x=None, v=None, a=None, f=None):
+        sh = (self.n, self.ndim)
+        arrs = [None if q is None else _f64(q, sh) for q in (x, v, a, f)]
+        _check(lib().parm_set_state(self.h, *[_d(q) for q in arrs]))
+
+    def state(self):
+        sh = (self.n, self.ndim)
+        x, v, a, f = (np.empty(sh) for _ in range(4))
+        _check(lib().parm_get_state(self.h, _d(x), _d(v), _d(a), _d(f)))
+        return x, v, a, f
+
+    def positions(self):
+        x = np.empty((self.n, self.ndim))
+        _check(lib().parm_get_state(self.h, _d(x), None, None, None))
+        return x
+
+    def forces(self):
+        f = np.empty((self.n, self.ndim))
+        _check(lib().parm_get_state(self.h, None, None, None, _d(f)))
+        return f
+
+    def set_masses(self, m):
+        _check(lib().parm_set_masses(self.h, _d(_f64(m).ravel())))
+
+    # -- neighbour list
+    def nlist_set(self, skin, diameters, members=None):
+        diam = _f64(diameters).ravel()
+        mem = np.arange(diam.size, dtype=np.int32) if members is None else \
+            np.ascontiguousarray(members, dtype=np.int32)
+        _check(lib().parm_nlist_set(self.h, float(skin), mem.size,
+                                    mem.ctypes.data_as(_ip), _d(diam)))
+
+    def update_list(self, force=True):
+        r = C.c_int(0)
+        _check(lib().parm_nlist_update(self.h, int(bool(force)), C.byref(r)))
+        return bool(r.value)
+
+    def which(self):
+        w = C.c_uint32(0)
+        _check(lib().parm_nlist_which(self.h, C.byref(w)))
+        return w.value
+
+    def numpairs(self):
+        n = C.c_int64(0)
+        _check(lib().parm_nlist_numpairs(self.h, C.byref(n)))
+        return n.value
+
+    def pairs(self):
+        n = self.numpairs()
+        first = np.empty(max(n, 1), np.int32)
+        last = np.empty(max(n, 1), np.int32)
+        got = C.c_int64(0)
+        _check(lib().parm_nlist_pairs(self.h, n, first.ctypes.data_as(_ip),
+                                      last.ctypes.data_as(_ip), C.byref(got)))
+        return first[:n], last[:n]
+
+    # -- interactions
+    def set_lj(self, eps_rows, sigma, sigcut, types):
+        sigma = _f64(sigma).ravel()
+        nt = sigma.size
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        _check(lib().parm_pair_lj_attract_repulse(
+            self.h, nt, _d(_f64(eps_rows, (nt, nt))), _d(sigma), _d(_f64(sigcut).ravel()),
+            t.ctypes.data_as(_ip)))
+
+    def set_repulsion(self, eps, sigma, exponent, types):
+        sigma = _f64(sigma).ravel()
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        _check(lib().parm_pair_repulsion(self.h, sigma.size, _d(_f64(eps).ravel()),
+                                         _d(sigma), _d(_f64(exponent).ravel()),
+                                         t.ctypes.data_as(_ip)))
+
+    def set_forces(self, and_a=True):
+        _check(lib().parm_set_forces(self.h, int(bool(and_a))))
+
+    def pair_sums(self):
+        U, W = C.c_double(0), C.c_double(0)
+        S = np.zeros((self.ndim, self.ndim))
+        nc, no = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().parm_pair_sums(self.h, C.byref(U), C.byref(W), _d(S), C.byref(nc),
+                                    C.byref(no)))
+        return dict(U=U.value, W=W.value, S=S, contacts=nc.value, overlaps=no.value)
+
+    def kinetic_energy(self, originv=None):
+        K = C.c_double(0)
+        ov = None if originv is None else _f64(originv).ravel()
+        _check(lib().parm_kinetic_energy(self.h, _d(ov), C.byref(K)))
+        return K.value
+
+    def momentum(self):
+        p = np.zeros(self.ndim)
+        m = C.c_double(0)
+        _check(lib().parm_momentum(self.h, _d(p), C.byref(m)))
+        return p, m.value
+
+    def force_stats(self):
+        fmax, fmean = C.c_double(0), C.c_double(0)
+        _check(lib().parm_force_stats(self.h, C.byref(fmax), C.byref(fmean)))
+        return fmax.value, fmean.value
+
+    # -- integrators
+    def initialize(self):
+        _check(lib().parm_initialize(self.h))
+
+    def verlet(self, dt, nsteps=1):
+        _check(lib().parm_verlet_steps(self.h, float(dt), int(nsteps)))
+
+    def damped(self, dt, damping, nsteps=1):
+        _check(lib().parm_damped_steps(self.h, float(dt), float(damping), int(nsteps)))
+
+    def synchronize(self):
+        _check(lib().parm_synchronize(self.h))
+
+    def profile_enable(self, on=True):
+        _check(lib().parm_profile_enable(self.h, int(bool(on))))
+
+    def profile_read(self, reset=True):
+        sm, rm = C.c_double(0), C.c_double(0)
+        sn, rn = C.c_int64(0), C.c_int64(0)
+        _check(lib().parm_profile_read(self.h, int(bool(reset)), C.byref(sm), C.byref(sn),
+                                       C.byref(rm), C.byref(rn)))
+        return dict(step_ms=sm.value, step_launches=sn.value, rebuild_ms=rm.value,
+                    rebuilds=rn.value)

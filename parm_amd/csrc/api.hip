@@ -1,0 +1,1146 @@
+// api.hip -- C ABI of libparm_hip.so (declared in include/parm_hip.h).
+//
+// Host orchestration of the device path: owns the per-system device state
+// (structure-of-arrays, spatially sorted slots), decides list rebuilds with
+// the reference's skin rule, and runs the step loop.  No CPU compute path
+// exists: every observable is computed on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/parm_hip.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace parm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PARM_HIP(call)                                                         \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return fail(PARM_E_DEVICE, std::string("HIP error ") +             \
+                                           hipGetErrorString(e_) + " in " +    \
+                                           #call);                             \
+    } while (0)
+
+#define PARM_CHECK(expr)                                                       \
+    do {                                                                       \
+        int r_ = (expr);                                                       \
+        if (r_ != PARM_OK) return r_;                                          \
+    } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(PARM_E_NOMEM, "device allocation of " +
+                                      std::to_string(sizeof(T) * count) +
+                                      " bytes failed");
+    }
+    return PARM_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+constexpr int kFlagRing = 4;
+// flag words: [0, kFlagRing) step ring, then the list-build overflow flag, a
+// scratch word for reductions that raise no flag, and the on-demand skin test
+constexpr int kSlotBuild = kFlagRing + 1, kSlotScratch = kFlagRing + 2,
+              kSlotSkin = kFlagRing + 3;
+
+struct parm_system {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ndim = 3, prec = kFP64;
+    int n = 0;
+    size_t pos_bytes = 32;  // sizeof(PosStore<D>::T)
+
+    Box box{};
+    bool box_set = false;
+
+    // per-slot device arrays (slot order = spatial sort order)
+    void *pos = nullptr, *pos_alt = nullptr, *xref = nullptr;
+    double *v[3] = {nullptr, nullptr, nullptr};
+    double *a[3] = {nullptr, nullptr, nullptr};
+    double *f[3] = {nullptr, nullptr, nullptr};
+    double *m = nullptr, *diam = nullptr;
+    int32_t *type = nullptr, *orig = nullptr, *rank = nullptr;
+    // permutation scratch
+    double *spare_d = nullptr;
+    int32_t *spare_i = nullptr;
+    // staging for uploads/downloads (n x ndim doubles, original order)
+    double *stage = nullptr;
+    double *stage_small = nullptr;  // per-atom scalars / int32 in original order
+
+    // neighbour list
+    bool list_cfg = false;
+    double skin = 0.0, dmax = 0.0;
+    int64_t nmem = 0;
+    bool ignorechanged = true;
+    uint32_t which = 0;
+    int64_t npairs = 0;
+    bool built = false;
+    CellGrid grid{};
+    int cap = 0, npad = 0, type_bits = 0;
+    int tpa = 4;            // lanes per atom in the force kernel (autotuned)
+    bool tpa_fixed = false, tuned = false;
+    uint32_t *nbr = nullptr;
+    int32_t *ncount = nullptr;
+    uint32_t *keys = nullptr, *tkeys = nullptr, *hist = nullptr;
+    int32_t *vals = nullptr, *tvals = nullptr, *cstart = nullptr;
+    int cstart_cap = 0;
+    unsigned long long *stats = nullptr, *bpart = nullptr;
+
+    // pair interaction
+    int pair_kind = kPairNone;
+    int ntypes = 0;
+    int single = 1, harmonic = 0;
+    PairParam *tab = nullptr;
+
+    // reductions and flags
+    double *part = nullptr, *out = nullptr;
+    uint32_t *dflag = nullptr;      // flag words (see kSlot*)
+    uint32_t *hflag = nullptr;      // pinned mirror
+    double *hout = nullptr;         // pinned mirror of out
+    hipEvent_t ev_flag = nullptr;
+    hipEvent_t ev_ring[kFlagRing] = {};
+
+    // instrumentation: event pairs around step-kernel launches and rebuilds
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_step, prof_rebuild;
+
+    AtomArrays arrays(bool alt_out = false) const {
+        AtomArrays A{};
+        A.n = n;
+        A.pos_in = pos;
+        A.pos_out = alt_out ? pos_alt : pos;
+        for (int k = 0; k < 3; k++) {
+            A.v[k] = v[k];
+            A.a[k] = a[k];
+            A.f[k] = f[k];
+        }
+        A.m = m;
+        A.type = type;
+        A.rank = rank;
+        A.xref = xref;
+        return A;
+    }
+    ListView listview() const {
+        ListView L{};
+        L.nbr = nbr;
+        L.ncount = ncount;
+        L.npad = npad;
+        L.type_bits = type_bits;
+        L.tpa = tpa;
+        L.built = built ? 1 : 0;
+        return L;
+    }
+    PairView pairview() const {
+        PairView P{};
+        P.kind = pair_kind;
+        P.ntypes = ntypes;
+        P.single = single;
+        P.harmonic = harmonic;
+        P.tab = tab;
+        return P;
+    }
+    ReduceBuf redbuf(int flag_index) const {
+        ReduceBuf R{};
+        R.part = part;
+        R.out = out;
+        R.flag = dflag + flag_index;
+        R.skin = skin;
+        return R;
+    }
+};
+
+namespace {
+
+int check_sys(const parm_system *s) {
+    if (!s) return fail(PARM_E_INVALID, "null parm_system");
+    return PARM_OK;
+}
+
+int set_device(const parm_system *s) {
+    PARM_HIP(hipSetDevice(s->device));
+    return PARM_OK;
+}
+
+void free_list(parm_system *s) {
+    dfree(s->nbr);
+    dfree(s->ncount);
+    s->cap = 0;
+}
+
+// Cell grid for the current box / diameters / skin: edge >= largest list
+// radius (dmax + skin) with a 1e-9 relative margin for rounding in the
+// binning; fewer than 3 cells along a dimension collapses it to 1 cell.
+void make_grid(parm_system *s) {
+    CellGrid g{};
+    const double rmax = (s->dmax + s->skin) * (1.0 + 1e-9);
+    long total = 1;
+    for (int k = 0; k < 3; k++) {
+        g.nc[k] = 1;
+        g.invL[k] = 0.0;
+    }
+    for (int k = 0; k < s->ndim; k++) {
+        double q = rmax > 0 ? std::floor(s->box.L[k] / rmax) : 1.0;
+        int nc = (q >= 3.0) ? (int)std::min(q, 4096.0) : 1;
+        g.nc[k] = nc;
+        g.invL[k] = s->box.invL[k];
+    }
+    // keep the grid within ~4 cells per atom (and at least 27 cells)
+    for (;;) {
+        total = 1;
+        for (int k = 0; k < s->ndim; k++) total *= g.nc[k];
+        if (total <= std::max<long>(4L * s->n, 27L)) break;
+        int kk = 0;
+        for (int k = 1; k < s->ndim; k++)
+            if (g.nc[k] > g.nc[kk]) kk = k;
+        g.nc[kk] = g.nc[kk] / 2 >= 3 ? g.nc[kk] / 2 : 1;
+    }
+    g.ncell = (int)total;
+    s->grid = g;
+}
+
+int ensure_list_capacity(parm_system *s, int cap) {
+    if (cap <= s->cap && s->nbr) return PARM_OK;
+    free_list(s);
+    s->cap = (cap + 7) & ~7;
+    PARM_CHECK(dalloc(&s->nbr, (size_t)s->cap * s->npad));
+    PARM_CHECK(dalloc(&s->ncount, (size_t)s->npad));
+    return PARM_OK;
+}
+
+int initial_capacity(const parm_system *s) {
+    double vol = 1.0;
+    for (int k = 0; k < s->ndim; k++) vol *= s->box.L[k];
+    const double r = s->dmax + s->skin;
+    const double ball = s->ndim == 3 ? 4.0 / 3.0 * M_PI * r * r * r : M_PI * r * r;
+    double mean = vol > 0 ? (double)s->nmem / vol * ball : 0.0;
+    double cap = 1.35 * mean + 24.0;
+    if (cap > (double)s->nmem) cap = (double)std::max<int64_t>(s->nmem, 8);
+    return std::max(16, (int)cap);
+}
+
+// Permute every per-slot array by perm (new slot i <- old slot perm[i]).
+int permute_all(parm_system *s, const int32_t *perm) {
+    hipStream_t st = s->stream;
+    const int n = s->n;
+    // positions: both buffers (the alternate one holds the next step's
+    // drifted positions inside a fused step loop); xref is the scratch, it is
+    // rewritten by the list build
+    for (void **pb : {&s->pos, &s->pos_alt}) {
+        if (s->ndim == 3) {
+            PARM_HIP(gather<double4>(n, perm, (const double4 *)*pb, (double4 *)s->xref, st));
+        } else {
+            PARM_HIP(gather<double2>(n, perm, (const double2 *)*pb, (double2 *)s->xref, st));
+        }
+        std::swap(*pb, s->xref);
+    }
+    double **dcols[] = {&s->v[0], &s->v[1], &s->v[2], &s->a[0], &s->a[1], &s->a[2],
+                        &s->f[0], &s->f[1], &s->f[2], &s->m,    &s->diam};
+    for (double **c : dcols) {
+        if (!*c) continue;
+        PARM_HIP(gather<double>(n, perm, *c, s->spare_d, st));
+        std::swap(*c, s->spare_d);
+    }
+    int32_t **icols[] = {&s->type, &s->orig, &s->rank};
+    for (int32_t **c : icols) {
+        PARM_HIP(gather<int32_t>(n, perm, *c, s->spare_i, st));
+        std::swap(*c, s->spare_i);
+    }
+    return PARM_OK;
+}
+
+// NeighborList::update_list's rebuild branch (src/trackers.cpp:55-69):
+// lastlocs = x, then the full pair list from the predicate.
+int rebuild(parm_system *s) {
+    if (!s->list_cfg) return fail(PARM_E_STATE, "neighbour list not configured");
+    hipStream_t st = s->stream;
+    const int n = s->n;
+    make_grid(s);
+    if (s->grid.ncell + 1 > s->cstart_cap) {
+        dfree(s->cstart);
+        s->cstart_cap = s->grid.ncell + 1;
+        PARM_CHECK(dalloc(&s->cstart, (size_t)s->cstart_cap));
+    }
+    if (n > 0) {
+        PARM_HIP(launch_cell_keys(s->ndim, n, s->pos, s->rank, s->grid, s->keys, s->vals,
+                                  st));
+        int bits = 1;
+        while ((1u << bits) <= (uint32_t)s->grid.ncell && bits < 32) bits++;
+        PARM_HIP(radix_sort(n, bits, s->keys, s->vals, s->tkeys, s->tvals, s->hist, st));
+        PARM_CHECK(permute_all(s, s->vals));
+    }
+    PARM_HIP(launch_cell_bounds(n, s->grid.ncell, s->keys, s->cstart, st));
+    if (s->cap == 0) PARM_CHECK(ensure_list_capacity(s, initial_capacity(s)));
+    for (int attempt = 0; attempt < 4; attempt++) {
+        PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
+        PARM_HIP(hipMemsetAsync(s->dflag + kSlotBuild, 0, sizeof(uint32_t), st));
+        PARM_HIP(launch_build_list(s->ndim, n, s->pos, s->rank, s->diam, s->type, s->keys,
+                                   s->cstart, s->grid, s->box, s->skin, s->type_bits,
+                                   s->tpa, s->cap, s->npad, s->nbr, s->ncount, s->xref,
+                                   s->bpart, s->stats, s->dflag + kSlotBuild, st));
+        unsigned long long hs[2] = {0, 0};
+        PARM_HIP(hipMemcpyAsync(hs, s->stats, sizeof(hs), hipMemcpyDeviceToHost, st));
+        PARM_HIP(hipStreamSynchronize(st));
+        if (hs[1] <= (unsigned long long)s->cap) {
+            s->npairs = (int64_t)(hs[0] / 2);
+            s->built = true;
+            s->which++;
+            s->ignorechanged = false;
+            return PARM_OK;
+        }
+        PARM_CHECK(ensure_list_capacity(s, (int)(hs[1] + hs[1] / 8 + 8)));
+    }
+    return fail(PARM_E_DEVICE, "neighbour list build did not converge");
+}
+
+int read_flag(parm_system *s, int idx, uint32_t *flag) {
+    PARM_HIP(hipMemcpyAsync(s->hflag + idx, s->dflag + idx, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s->stream));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    *flag = s->hflag[idx];
+    return PARM_OK;
+}
+
+int update_list(parm_system *s, bool force, int *rebuilt) {
+    if (rebuilt) *rebuilt = 0;
+    if (!s->list_cfg) return fail(PARM_E_STATE, "neighbour list not configured");
+    if (!force && !s->ignorechanged) {
+        PARM_HIP(hipMemsetAsync(s->dflag + kSlotSkin, 0, sizeof(uint32_t), s->stream));
+        PARM_HIP(launch_skin(s->ndim, s->arrays(), s->redbuf(kSlotSkin), s->stream));
+        uint32_t fl = 0;
+        PARM_CHECK(read_flag(s, kSlotSkin, &fl));
+        if (fl & kFlagNonFinite)
+            return fail(PARM_E_NONFINITE, "non-finite position in skin test");
+        if (!(fl & kFlagRebuild)) return PARM_OK;
+    }
+    PARM_CHECK(rebuild(s));
+    if (rebuilt) *rebuilt = 1;
+    return PARM_OK;
+}
+
+int require_ready(parm_system *s) {
+    if (!s->box_set) return fail(PARM_E_STATE, "box not set (parm_set_box)");
+    if (!s->list_cfg) return fail(PARM_E_STATE, "neighbour list not set (parm_nlist_set)");
+    if (s->pair_kind == kPairNone)
+        return fail(PARM_E_STATE, "no pair interaction set");
+    return PARM_OK;
+}
+
+int set_forces(parm_system *s, bool and_a) {
+    StepCoef c{};
+    PARM_HIP(launch_force(s->ndim, s->prec, and_a ? kStageForceA : kStageForceOnly, false,
+                          s->arrays(), s->listview(), s->pairview(), s->box, c,
+                          s->redbuf(kSlotScratch), s->stream));
+    return PARM_OK;
+}
+
+hipEvent_t pool_event(parm_system *s) {
+    if (s->ev_used == s->ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        s->ev_pool.push_back(e);
+    }
+    return s->ev_pool[s->ev_used++];
+}
+
+// Brackets a launch with timing events when instrumentation is on.
+struct ProfScope {
+    parm_system *s;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> *dst;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(parm_system *s_, std::vector<std::pair<hipEvent_t, hipEvent_t>> *d)
+        : s(s_), dst(d) {
+        if (!s->prof) return;
+        a = pool_event(s);
+        b = pool_event(s);
+        if (a && b) (void)hipEventRecord(a, s->stream);
+    }
+    ~ProfScope() {
+        if (!s->prof || !a || !b) return;
+        (void)hipEventRecord(b, s->stream);
+        dst->emplace_back(a, b);
+    }
+};
+
+// Rewrites the current list in the layout of s->tpa without changing its
+// pairs: the build runs again on the positions of the last rebuild (xref),
+// with the cell keys and bounds of that rebuild.
+int relayout(parm_system *s) {
+    if (!s->built || s->n == 0) return PARM_OK;
+    hipStream_t st = s->stream;
+    PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
+    PARM_HIP(launch_build_list(s->ndim, s->n, s->xref, s->rank, s->diam, s->type, s->keys,
+                               s->cstart, s->grid, s->box, s->skin, s->type_bits, s->tpa,
+                               s->cap, s->npad, s->nbr, s->ncount, s->xref, s->bpart,
+                               s->stats, s->dflag + kSlotBuild, st));
+    return PARM_OK;
+}
+
+// Autotuner for the lanes-per-atom of the force kernel: each candidate lays
+// the list out for itself and times three force-only launches with events;
+// the fastest is kept for the life of the system.  Force-only launches only
+// rewrite f at the current positions, which the next step overwrites.
+int autotune(parm_system *s) {
+    if (s->tuned || s->tpa_fixed || !s->built || s->n == 0) return PARM_OK;
+    hipStream_t st = s->stream;
+    hipEvent_t e0 = pool_event(s), e1 = pool_event(s);
+    if (!e0 || !e1) return fail(PARM_E_DEVICE, "event creation failed");
+    StepCoef c{};
+    int best = s->tpa;
+    float best_ms = 1e30f;
+    for (int tpa : {1, 2, 4, 8}) {
+        s->tpa = tpa;
+        PARM_CHECK(relayout(s));
+        PARM_HIP(launch_force(s->ndim, s->prec, kStageForceOnly, false, s->arrays(),
+                              s->listview(), s->pairview(), s->box, c,
+                              s->redbuf(kSlotScratch), st));
+        float tmin = 1e30f;
+        for (int r = 0; r < 4; r++) {  // best of 4 single launches
+            PARM_HIP(hipEventRecord(e0, st));
+            PARM_HIP(launch_force(s->ndim, s->prec, kStageForceOnly, false, s->arrays(),
+                                  s->listview(), s->pairview(), s->box, c,
+                                  s->redbuf(kSlotScratch), st));
+            PARM_HIP(hipEventRecord(e1, st));
+            PARM_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            PARM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            tmin = std::min(tmin, ms);
+        }
+        if (tmin < best_ms * 0.97f) {  // prefer the earlier (smaller) on ties
+            best_ms = tmin;
+            best = tpa;
+        }
+    }
+    s->ev_used -= 2;
+    s->tpa = best;
+    PARM_CHECK(relayout(s));
+    s->tuned = true;
+    return PARM_OK;
+}
+
+// Copies flag slot q to pinned memory and records its ring event.
+int post_flag(parm_system *s, int slot) {
+    PARM_HIP(hipMemcpyAsync(s->hflag + slot, s->dflag + slot, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s->stream));
+    PARM_HIP(hipEventRecord(s->ev_ring[slot], s->stream));
+    return PARM_OK;
+}
+
+// nsteps x timestep() of CollectionVerlet / CollectionDamped, fused.
+//
+// The reference step is: drift (first loop), set_forces, kick (second loop),
+// update_trackers (skin test -> maybe rebuild).  Here one kernel per step does
+// the force + kick of step q AND the drift of step q+1 into the alternate
+// position buffer, plus the skin test of those drifted positions.  The list
+// decision for the end of step q needs the skin test of x_{q+1}, which the
+// previous kernel produced: the host waits for that flag while the current
+// kernel runs.  A rebuild at x_{q+1} permutes both position buffers and
+// re-tests x_{q+2} against the new reference positions.  The last step runs
+// force + kick only, so the state returned is at a full step.
+int run_steps(parm_system *s, bool damped, const StepCoef &c, int64_t nsteps) {
+    PARM_CHECK(require_ready(s));
+    if (nsteps <= 0) return PARM_OK;
+    if (nsteps >= 16) PARM_CHECK(autotune(s));
+    hipStream_t st = s->stream;
+    const int kick = damped ? kStageDampedKick : kStageVerletKick;
+    const int kick_drift = damped ? kStageDampedKickDrift : kStageVerletKickDrift;
+    // first half of step 0
+    PARM_HIP(hipMemsetAsync(s->dflag, 0, sizeof(uint32_t), st));
+    PARM_HIP(launch_drift(s->ndim, kick, s->arrays(), c, s->redbuf(0), st));
+    PARM_CHECK(post_flag(s, 0));
+    for (int64_t q = 0; q < nsteps; q++) {
+        const bool last = (q == nsteps - 1);
+        const int slot = (int)(q % kFlagRing), next = (int)((q + 1) % kFlagRing);
+        if (!last) {
+            PARM_HIP(hipMemsetAsync(s->dflag + next, 0, sizeof(uint32_t), st));
+            {
+                ProfScope ps(s, &s->prof_step);
+                PARM_HIP(launch_force(s->ndim, s->prec, kick_drift, false,
+                                      s->arrays(true), s->listview(), s->pairview(),
+                                      s->box, c, s->redbuf(next), st));
+            }
+            PARM_CHECK(post_flag(s, next));
+        } else {
+            PARM_HIP(launch_force(s->ndim, s->prec, kick, false, s->arrays(),
+                                  s->listview(), s->pairview(), s->box, c,
+                                  s->redbuf(kSlotScratch), st));
+        }
+        // update_trackers at the end of step q: skin test of x_{q+1}
+        PARM_HIP(hipEventSynchronize(s->ev_ring[slot]));
+        const uint32_t fl = s->hflag[slot];
+        if (fl & kFlagNonFinite)
+            return fail(PARM_E_NONFINITE,
+                        "position became non-finite at step " + std::to_string(q));
+        if ((fl & kFlagRebuild) || s->ignorechanged) {
+            {
+                ProfScope ps(s, &s->prof_rebuild);
+                PARM_CHECK(rebuild(s));
+            }
+            if (!last) {
+                PARM_HIP(hipMemsetAsync(s->dflag + next, 0, sizeof(uint32_t), st));
+                AtomArrays A = s->arrays();
+                A.pos_in = s->pos_alt;
+                PARM_HIP(launch_skin(s->ndim, A, s->redbuf(next), st));
+                PARM_CHECK(post_flag(s, next));
+            }
+        }
+        if (!last) std::swap(s->pos, s->pos_alt);
+    }
+    return PARM_OK;
+}
+
+int upload_vec(parm_system *s, int what, const double *host) {
+    const size_t bytes = sizeof(double) * (size_t)s->n * s->ndim;
+    if (bytes == 0) return PARM_OK;
+    PARM_HIP(hipMemcpyAsync(s->stage, host, bytes, hipMemcpyHostToDevice, s->stream));
+    PARM_HIP(launch_upload(s->ndim, what, s->arrays(), s->orig, s->stage, s->stream));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    return PARM_OK;
+}
+
+int download_vec(parm_system *s, int what, double *host) {
+    const size_t bytes = sizeof(double) * (size_t)s->n * s->ndim;
+    if (bytes == 0) return PARM_OK;
+    PARM_HIP(launch_download(s->ndim, what, s->arrays(), s->orig, s->stage, s->stream));
+    PARM_HIP(hipMemcpyAsync(host, s->stage, bytes, hipMemcpyDeviceToHost, s->stream));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    return PARM_OK;
+}
+
+// Per-atom scalar in original order -> slot order.
+template <typename T>
+int upload_scalar(parm_system *s, const T *host, T *dst) {
+    if (s->n == 0) return PARM_OK;
+    static_assert(sizeof(T) <= sizeof(double), "scalar");
+    PARM_HIP(hipMemcpyAsync(s->stage_small, host, sizeof(T) * s->n,
+                            hipMemcpyHostToDevice, s->stream));
+    PARM_HIP(gather<T>(s->n, s->orig, (const T *)s->stage_small, dst, s->stream));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    return PARM_OK;
+}
+
+int read_out(parm_system *s) {
+    PARM_HIP(hipMemcpyAsync(s->hout, s->out, sizeof(double) * kRedWidth,
+                            hipMemcpyDeviceToHost, s->stream));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    return PARM_OK;
+}
+
+void destroy(parm_system *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    dfree(s->pos);
+    dfree(s->pos_alt);
+    dfree(s->xref);
+    for (int k = 0; k < 3; k++) {
+        dfree(s->v[k]);
+        dfree(s->a[k]);
+        dfree(s->f[k]);
+    }
+    dfree(s->m);
+    dfree(s->diam);
+    dfree(s->type);
+    dfree(s->orig);
+    dfree(s->rank);
+    dfree(s->spare_d);
+    dfree(s->spare_i);
+    dfree(s->stage);
+    dfree(s->stage_small);
+    free_list(s);
+    dfree(s->keys);
+    dfree(s->tkeys);
+    dfree(s->hist);
+    dfree(s->vals);
+    dfree(s->tvals);
+    dfree(s->cstart);
+    dfree(s->stats);
+    dfree(s->bpart);
+    dfree(s->tab);
+    dfree(s->part);
+    dfree(s->out);
+    dfree(s->dflag);
+    if (s->hflag) (void)hipHostFree(s->hflag);
+    if (s->hout) (void)hipHostFree(s->hout);
+    if (s->ev_flag) (void)hipEventDestroy(s->ev_flag);
+    for (hipEvent_t &e : s->ev_ring)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+__global__ void k_iota(int n, int32_t *p) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+__global__ void k_fill_i32(int n, int32_t *p, int32_t v) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int build_pair_table(parm_system *s, int kind, int ntypes, const double *eps,
+                     const double *sigma, const double *third, const int32_t *atype) {
+    if (ntypes < 1 || ntypes > 64)
+        return fail(PARM_E_UNSUPPORTED, "ntypes must be in [1, 64]");
+    if (!eps || !sigma || !third || (!atype && s->n > 0))
+        return fail(PARM_E_INVALID, "null parameter array");
+    for (int i = 0; i < s->n; i++)
+        if (atype[i] < 0 || atype[i] >= ntypes)
+            return fail(PARM_E_INVALID, "atom type out of range at atom " + std::to_string(i));
+    std::vector<PairParam> tab((size_t)ntypes * ntypes);
+    bool harm = true;
+    for (int t1 = 0; t1 < ntypes; t1++) {
+        for (int t2 = 0; t2 < ntypes; t2++) {
+            PairParam p{};
+            if (kind == kPairLJ) {
+                // LJAttractRepulsePair(a1, a2)  (interaction.hpp:1256-1269)
+                double e = eps[(size_t)t1 * ntypes + t2];
+                double sig = (sigma[t1] + sigma[t2]) / 2.0;
+                double cut = std::max(third[t1], third[t2]);
+                double cut_energy;
+                if (e <= 0) {
+                    cut = 1;
+                    cut_energy = 0;
+                    e = std::fabs(e);
+                } else {
+                    double mid = (1 - std::pow(cut, -6));
+                    cut_energy = e * (mid * mid);
+                }
+                p.eps = e;
+                p.sig = sig;
+                p.sig2 = sig * sig;
+                p.cut2 = cut * cut;
+                p.lim_lo = p.sig2 * p.cut2 * (1.0 - 1e-14);
+                p.lim_hi = p.sig2 * p.cut2 * (1.0 + 1e-14);
+                p.p5 = cut_energy;
+                p.p6 = 12 * e;
+                p.lim_f = (float)(p.sig2 * p.cut2);
+            } else {
+                // RepulsionPair(a1, a2)  (interaction.hpp:1532-1537)
+                double e = std::sqrt(eps[t1] * eps[t2]);
+                double sig = (sigma[t1] + sigma[t2]) / 2.0;
+                double nexp = (third[t1] + third[t2]) / 2.0;
+                p.eps = e;
+                p.sig = sig;
+                p.sig2 = sig * sig;
+                p.cut2 = p.sig2;
+                p.lim_lo = p.sig2;
+                p.lim_hi = p.sig2;
+                p.p5 = nexp;
+                p.p6 = nexp - 1;
+                p.lim_f = (float)p.sig2;
+                if (nexp != 2.0) harm = false;
+            }
+            p.eps_f = (float)p.eps;
+            p.sig2_f = (float)p.sig2;
+            p.p5_f = (float)p.p5;
+            p.p6_f = (float)p.p6;
+            p.sig_f = (float)p.sig;
+            p.isig_f = (float)(1.0 / p.sig);
+            tab[(size_t)t1 * ntypes + t2] = p;
+        }
+    }
+    PARM_CHECK(set_device(s));
+    dfree(s->tab);
+    PARM_CHECK(dalloc(&s->tab, tab.size()));
+    PARM_HIP(hipMemcpy(s->tab, tab.data(), sizeof(PairParam) * tab.size(),
+                       hipMemcpyHostToDevice));
+    if (s->n > 0) PARM_CHECK(upload_scalar<int32_t>(s, atype, s->type));
+    int bits = 0;
+    while ((1 << bits) < ntypes) bits++;
+    if (bits > 0 && (int64_t)s->n >= (int64_t(1) << (32 - bits)))
+        return fail(PARM_E_UNSUPPORTED, "too many atoms for this many types");
+    s->type_bits = bits;
+    s->ntypes = ntypes;
+    s->single = ntypes == 1 ? 1 : 0;
+    s->harmonic = (kind == kPairRepulsion && harm) ? 1 : 0;
+    s->pair_kind = kind;
+    s->ignorechanged = true;  // list entries carry the type bits
+    return PARM_OK;
+}
+
+}  // namespace
+
+// ============================================================== the C ABI
+extern "C" {
+
+const char *parm_last_error(void) { return g_err.c_str(); }
+
+int parm_version(int *major, int *minor) {
+    if (major) *major = 1;
+    if (minor) *minor = 0;
+    return PARM_OK;
+}
+
+int parm_device_count(int *count) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        c = 0;
+    }
+    if (count) *count = c;
+    return PARM_OK;
+}
+
+int parm_system_create(int ndim, int precision, int64_t natoms, const double *masses,
+                       parm_system **out) {
+    if (!out) return fail(PARM_E_INVALID, "out is null");
+    *out = nullptr;
+    if (ndim != 2 && ndim != 3) return fail(PARM_E_INVALID, "ndim must be 2 or 3");
+    if (precision != PARM_FP64 && precision != PARM_MIXED)
+        return fail(PARM_E_INVALID, "precision must be PARM_FP64 or PARM_MIXED");
+    if (natoms < 0 || natoms >= (int64_t(1) << 31))
+        return fail(PARM_E_INVALID, "natoms out of range");
+    if (natoms > 0 && !masses) return fail(PARM_E_INVALID, "masses is null");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return fail(PARM_E_DEVICE, "no HIP device available (the device path has no CPU fallback)");
+    }
+    parm_system *s = new parm_system();
+    s->ndim = ndim;
+    s->prec = precision;
+    s->n = (int)natoms;
+    s->npad = (int)((natoms + 63) / 64 * 64);
+    if (s->npad == 0) s->npad = 64;
+    s->pos_bytes = ndim == 3 ? sizeof(double4) : sizeof(double2);
+    if (const char *e = std::getenv("PARM_TPA")) {
+        int t = std::atoi(e);
+        if (t == 1 || t == 2 || t == 4 || t == 8) {
+            s->tpa = t;
+            s->tpa_fixed = true;
+        }
+    }
+    int rc = PARM_OK;
+    auto bail = [&](int r) {
+        destroy(s);
+        return r;
+    };
+    if (hipGetDevice(&s->device) != hipSuccess) return bail(fail(PARM_E_DEVICE, "hipGetDevice failed"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, s->device) != hipSuccess)
+        return bail(fail(PARM_E_DEVICE, "hipGetDeviceProperties failed"));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return bail(fail(PARM_E_DEVICE, std::string("device is ") + prop.gcnArchName +
+                                            "; this library is built for gfx950 only"));
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(PARM_E_DEVICE, "stream creation failed"));
+    if (hipEventCreateWithFlags(&s->ev_flag, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(PARM_E_DEVICE, "event creation failed"));
+    for (hipEvent_t &e : s->ev_ring)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(PARM_E_DEVICE, "event creation failed"));
+    const size_t n = (size_t)std::max(1, s->n);
+    if ((rc = dalloc((char **)&s->pos, n * s->pos_bytes)) ||
+        (rc = dalloc((char **)&s->pos_alt, n * s->pos_bytes)) ||
+        (rc = dalloc((char **)&s->xref, n * s->pos_bytes)))
+        return bail(rc);
+    for (int k = 0; k < 3; k++) {
+        if ((rc = dalloc(&s->v[k], n)) || (rc = dalloc(&s->a[k], n)) ||
+            (rc = dalloc(&s->f[k], n)))
+            return bail(rc);
+    }
+    if ((rc = dalloc(&s->m, n)) || (rc = dalloc(&s->diam, n)) ||
+        (rc = dalloc(&s->type, n)) || (rc = dalloc(&s->orig, n)) ||
+        (rc = dalloc(&s->rank, n)) || (rc = dalloc(&s->spare_d, n)) ||
+        (rc = dalloc(&s->spare_i, n)) || (rc = dalloc(&s->stage, n * ndim)) ||
+        (rc = dalloc(&s->stage_small, n)) || (rc = dalloc(&s->keys, n)) ||
+        (rc = dalloc(&s->tkeys, n)) || (rc = dalloc(&s->vals, n)) ||
+        (rc = dalloc(&s->tvals, n)) ||
+        (rc = dalloc(&s->hist, (size_t)256 * radix_blocks((int)n))) ||
+        (rc = dalloc(&s->stats, 2)) ||
+        (rc = dalloc(&s->bpart, 2 * (size_t)build_blocks((int)n))) ||
+        (rc = dalloc(&s->part, (size_t)kRedWidth * force_blocks((int)n, 8))) ||
+        (rc = dalloc(&s->out, kRedWidth)) ||
+        (rc = dalloc(&s->dflag, 2 * kFlagRing + 4)))
+        return bail(rc);
+    if (hipHostMalloc((void **)&s->hflag, (2 * kFlagRing + 4) * sizeof(uint32_t), 0) != hipSuccess ||
+        hipHostMalloc((void **)&s->hout, kRedWidth * sizeof(double), 0) != hipSuccess)
+        return bail(fail(PARM_E_NOMEM, "pinned host allocation failed"));
+    hipStream_t st = s->stream;
+    const int nb = (int)((n + 255) / 256);
+    // zero state; orig = identity; no list members; type 0
+    for (int k = 0; k < 3; k++) {
+        if (hipMemsetAsync(s->v[k], 0, n * sizeof(double), st) != hipSuccess ||
+            hipMemsetAsync(s->a[k], 0, n * sizeof(double), st) != hipSuccess ||
+            hipMemsetAsync(s->f[k], 0, n * sizeof(double), st) != hipSuccess)
+            return bail(fail(PARM_E_DEVICE, "memset failed"));
+    }
+    (void)hipMemsetAsync(s->pos, 0, n * s->pos_bytes, st);
+    (void)hipMemsetAsync(s->pos_alt, 0, n * s->pos_bytes, st);
+    (void)hipMemsetAsync(s->xref, 0, n * s->pos_bytes, st);
+    (void)hipMemsetAsync(s->type, 0, n * sizeof(int32_t), st);
+    (void)hipMemsetAsync(s->diam, 0, n * sizeof(double), st);
+    (void)hipMemsetAsync(s->dflag, 0, (2 * kFlagRing + 4) * sizeof(uint32_t), st);
+    hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, st, (int)n, s->orig);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, (int)n, s->rank, -1);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return bail(fail(PARM_E_DEVICE, "initialisation kernels failed"));
+    if (s->n > 0) {
+        rc = upload_scalar<double>(s, masses, s->m);
+        if (rc) return bail(rc);
+    }
+    *out = s;
+    return PARM_OK;
+}
+
+int parm_system_destroy(parm_system *sys) {
+    destroy(sys);
+    return PARM_OK;
+}
+
+int parm_system_info(const parm_system *s, int *ndim, int *precision, int64_t *natoms) {
+    PARM_CHECK(check_sys(s));
+    if (ndim) *ndim = s->ndim;
+    if (precision) *precision = s->prec;
+    if (natoms) *natoms = s->n;
+    return PARM_OK;
+}
+
+int parm_set_box(parm_system *s, const double *L) {
+    PARM_CHECK(check_sys(s));
+    if (!L) return fail(PARM_E_INVALID, "L is null");
+    for (int k = 0; k < s->ndim; k++)
+        if (!(L[k] > 0) || !std::isfinite(L[k]))
+            return fail(PARM_E_INVALID, "box edges must be finite and > 0");
+    for (int k = 0; k < 3; k++) {
+        double Lk = k < s->ndim ? L[k] : 1.0;
+        s->box.L[k] = Lk;
+        s->box.invL[k] = 1.0 / Lk;
+        s->box.halfL[k] = Lk / 2;
+    }
+    s->box_set = true;
+    return PARM_OK;
+}
+
+int parm_set_state(parm_system *s, const double *x, const double *v, const double *a,
+                   const double *f) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(set_device(s));
+    if (x) PARM_CHECK(upload_vec(s, 0, x));
+    if (v) PARM_CHECK(upload_vec(s, 1, v));
+    if (a) PARM_CHECK(upload_vec(s, 2, a));
+    if (f) PARM_CHECK(upload_vec(s, 3, f));
+    return PARM_OK;
+}
+
+int parm_get_state(parm_system *s, double *x, double *v, double *a, double *f) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(set_device(s));
+    if (x) PARM_CHECK(download_vec(s, 0, x));
+    if (v) PARM_CHECK(download_vec(s, 1, v));
+    if (a) PARM_CHECK(download_vec(s, 2, a));
+    if (f) PARM_CHECK(download_vec(s, 3, f));
+    return PARM_OK;
+}
+
+int parm_set_masses(parm_system *s, const double *masses) {
+    PARM_CHECK(check_sys(s));
+    if (!masses && s->n > 0) return fail(PARM_E_INVALID, "masses is null");
+    PARM_CHECK(set_device(s));
+    return upload_scalar<double>(s, masses, s->m);
+}
+
+int parm_nlist_set(parm_system *s, double skin, int64_t nmembers, const int32_t *member,
+                   const double *diameter) {
+    PARM_CHECK(check_sys(s));
+    if (!(skin >= 0) || !std::isfinite(skin))
+        return fail(PARM_E_INVALID, "skin must be finite and >= 0");
+    if (nmembers < 0 || nmembers > s->n)
+        return fail(PARM_E_INVALID, "nmembers out of range");
+    if (nmembers > 0 && (!member || !diameter))
+        return fail(PARM_E_INVALID, "null member/diameter array");
+    std::vector<int32_t> rank(s->n, -1);
+    std::vector<double> dia(s->n, 0.0);
+    double dmax = 0.0;
+    for (int64_t k = 0; k < nmembers; k++) {
+        int32_t a = member[k];
+        if (a < 0 || a >= s->n)
+            return fail(PARM_E_INVALID, "member index out of range");
+        if (rank[a] >= 0)
+            return fail(PARM_E_INVALID,
+                        "Cannot add AtomID to SubGroup: it already exists.");
+        if (!(diameter[k] >= 0) || !std::isfinite(diameter[k]))
+            return fail(PARM_E_INVALID, "diameters must be finite and >= 0");
+        rank[a] = (int32_t)k;
+        dia[a] = diameter[k];
+        dmax = std::max(dmax, diameter[k]);
+    }
+    PARM_CHECK(set_device(s));
+    if (s->n > 0) {
+        PARM_CHECK(upload_scalar<int32_t>(s, rank.data(), s->rank));
+        PARM_CHECK(upload_scalar<double>(s, dia.data(), s->diam));
+    }
+    s->skin = skin;
+    s->dmax = dmax;
+    s->nmem = nmembers;
+    s->list_cfg = true;
+    s->ignorechanged = true;
+    s->built = false;
+    s->npairs = 0;
+    free_list(s);
+    return PARM_OK;
+}
+
+int parm_nlist_update(parm_system *s, int force, int *rebuilt) {
+    PARM_CHECK(check_sys(s));
+    if (!s->box_set) return fail(PARM_E_STATE, "box not set (parm_set_box)");
+    PARM_CHECK(set_device(s));
+    return update_list(s, force != 0, rebuilt);
+}
+
+int parm_nlist_which(parm_system *s, uint32_t *which) {
+    PARM_CHECK(check_sys(s));
+    if (which) *which = s->which;
+    return PARM_OK;
+}
+
+int parm_nlist_numpairs(parm_system *s, int64_t *npairs) {
+    PARM_CHECK(check_sys(s));
+    if (npairs) *npairs = s->built ? s->npairs : 0;
+    return PARM_OK;
+}
+
+int parm_nlist_pairs(parm_system *s, int64_t cap, int32_t *first, int32_t *last,
+                     int64_t *npairs) {
+    PARM_CHECK(check_sys(s));
+    if (cap > 0 && (!first || !last)) return fail(PARM_E_INVALID, "null output array");
+    PARM_CHECK(set_device(s));
+    const int64_t np = s->built ? s->npairs : 0;
+    if (npairs) *npairs = np;
+    if (np == 0 || cap <= 0) return PARM_OK;
+    // Export: read the full list back and keep, per member i, the entries of
+    // lower member rank, sorted by rank -- the reference's pair order.
+    const int n = s->n;
+    std::vector<int32_t> cnt(n), orig(n), rank(n);
+    std::vector<uint32_t> nb((size_t)s->cap * s->npad);
+    PARM_HIP(hipMemcpyAsync(cnt.data(), s->ncount, sizeof(int32_t) * n,
+                            hipMemcpyDeviceToHost, s->stream));
+    PARM_HIP(hipMemcpyAsync(orig.data(), s->orig, sizeof(int32_t) * n,
+                            hipMemcpyDeviceToHost, s->stream));
+    PARM_HIP(hipMemcpyAsync(rank.data(), s->rank, sizeof(int32_t) * n,
+                            hipMemcpyDeviceToHost, s->stream));
+    PARM_HIP(hipMemcpyAsync(nb.data(), s->nbr, sizeof(uint32_t) * nb.size(),
+                            hipMemcpyDeviceToHost, s->stream));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    std::vector<int32_t> slot_of_rank(s->nmem, -1);
+    for (int i = 0; i < n; i++)
+        if (rank[i] >= 0) slot_of_rank[rank[i]] = i;
+    int64_t w = 0;
+    std::vector<int32_t> lower;
+    for (int64_t r = 0; r < s->nmem; r++) {
+        const int i = slot_of_rank[r];
+        lower.clear();
+        for (int q = 0; q < cnt[i]; q++) {
+            const size_t idx = ((size_t)(q / s->tpa) * s->npad + i) * s->tpa + q % s->tpa;
+            const int j = (int)(nb[idx] >> s->type_bits);
+            if (rank[j] < r) lower.push_back(rank[j]);
+        }
+        std::sort(lower.begin(), lower.end());
+        for (int32_t rj : lower) {
+            if (w < cap) {
+                first[w] = orig[i];
+                last[w] = orig[slot_of_rank[rj]];
+            }
+            w++;
+        }
+    }
+    if (w != np)
+        return fail(PARM_E_DEVICE, "pair export count mismatch (list not symmetric)");
+    return PARM_OK;
+}
+
+int parm_pair_lj_attract_repulse(parm_system *s, int ntypes, const double *eps_rows,
+                                 const double *sigma, const double *sigcut,
+                                 const int32_t *atom_type) {
+    PARM_CHECK(check_sys(s));
+    return build_pair_table(s, kPairLJ, ntypes, eps_rows, sigma, sigcut, atom_type);
+}
+
+int parm_pair_repulsion(parm_system *s, int ntypes, const double *eps,
+                        const double *sigma, const double *exponent,
+                        const int32_t *atom_type) {
+    PARM_CHECK(check_sys(s));
+    return build_pair_table(s, kPairRepulsion, ntypes, eps, sigma, exponent, atom_type);
+}
+
+int parm_set_forces(parm_system *s, int and_a) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(require_ready(s));
+    PARM_CHECK(set_device(s));
+    return set_forces(s, and_a != 0);
+}
+
+int parm_pair_sums(parm_system *s, double *U, double *W, double *S, uint64_t *contacts,
+                   uint64_t *overlaps) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(require_ready(s));
+    PARM_CHECK(set_device(s));
+    StepCoef c{};
+    PARM_HIP(launch_force(s->ndim, s->prec, kStageSums, true, s->arrays(), s->listview(),
+                          s->pairview(), s->box, c, s->redbuf(kSlotScratch), s->stream));
+    PARM_CHECK(read_out(s));
+    const double *o = s->hout;
+    if (U) *U = o[0];
+    if (W) *W = o[1];
+    if (S)
+        for (int k = 0; k < s->ndim * s->ndim; k++) S[k] = o[2 + k];
+    if (contacts) *contacts = (uint64_t)std::llround(o[11] / 2);
+    if (overlaps) *overlaps = (uint64_t)std::llround(o[12] / 2);
+    return PARM_OK;
+}
+
+int parm_kinetic_energy(parm_system *s, const double *originv, double *K) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(set_device(s));
+    ReduceBuf R = s->redbuf(kSlotScratch);
+    R.skin = INFINITY;
+    PARM_HIP(launch_atom_sums(s->ndim, 0, s->arrays(), originv, R, s->stream));
+    PARM_CHECK(read_out(s));
+    if (K) *K = s->hout[0];
+    return PARM_OK;
+}
+
+int parm_momentum(parm_system *s, double *p, double *mass) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(set_device(s));
+    ReduceBuf R = s->redbuf(kSlotScratch);
+    R.skin = INFINITY;
+    PARM_HIP(launch_atom_sums(s->ndim, 0, s->arrays(), nullptr, R, s->stream));
+    PARM_CHECK(read_out(s));
+    if (p)
+        for (int k = 0; k < s->ndim; k++) p[k] = s->hout[1 + k];
+    if (mass) *mass = s->hout[4];
+    return PARM_OK;
+}
+
+int parm_force_stats(parm_system *s, double *fmax, double *fmean) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(set_device(s));
+    ReduceBuf R = s->redbuf(kSlotScratch);
+    R.skin = INFINITY;
+    PARM_HIP(launch_atom_sums(s->ndim, 1, s->arrays(), nullptr, R, s->stream));
+    PARM_CHECK(read_out(s));
+    if (fmax) *fmax = s->hout[13];
+    if (fmean) *fmean = s->n > 0 ? s->hout[0] / s->n : 0.0;
+    return PARM_OK;
+}
+
+int parm_initialize(parm_system *s) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(require_ready(s));
+    PARM_CHECK(set_device(s));
+    PARM_CHECK(update_list(s, false, nullptr));
+    PARM_CHECK(set_forces(s, true));
+    PARM_CHECK(update_list(s, false, nullptr));
+    return PARM_OK;
+}
+
+int parm_verlet_steps(parm_system *s, double dt, int64_t nsteps) {
+    PARM_CHECK(check_sys(s));
+    if (nsteps < 0) return fail(PARM_E_INVALID, "nsteps < 0");
+    PARM_CHECK(set_device(s));
+    StepCoef c{};
+    c.k0 = dt;
+    c.k1 = dt * dt / 2;
+    c.k2 = dt / 2;
+    return run_steps(s, false, c, nsteps);
+}
+
+int parm_damped_steps(parm_system *s, double dt, double damping, int64_t nsteps) {
+    PARM_CHECK(check_sys(s));
+    if (dt <= 0) return fail(PARM_E_INVALID, "Collection::CollectionSol: dt >= 0");
+    if (nsteps < 0) return fail(PARM_E_INVALID, "nsteps < 0");
+    PARM_CHECK(set_device(s));
+    // CollectionDamped::set_constants (src/collection.cpp:342-354)
+    double c0, c1, c2;
+    if (damping <= 0.0) {
+        c0 = 1;
+        c1 = 1;
+        c2 = .5;
+    } else {
+        double dampdt = damping * dt;
+        c0 = std::exp(-dampdt);
+        c1 = (-std::expm1(-dampdt)) / dampdt;
+        c2 = (1 - c1) / dampdt;
+    }
+    StepCoef c{};
+    c.k0 = c1 * dt;
+    c.k1 = c2 * dt * dt;
+    c.k2 = dt * (c1 - c2);
+    c.k3 = c0;
+    c.k4 = dt * c2;
+    return run_steps(s, true, c, nsteps);
+}
+
+int parm_profile_enable(parm_system *s, int enable) {
+    PARM_CHECK(check_sys(s));
+    s->prof = enable != 0;
+    return PARM_OK;
+}
+
+int parm_profile_read(parm_system *s, int reset, double *step_ms, int64_t *step_launches,
+                      double *rebuild_ms, int64_t *rebuilds) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(set_device(s));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    auto total = [](const std::vector<std::pair<hipEvent_t, hipEvent_t>> &v, double *ms) {
+        double t = 0.0;
+        for (const auto &p : v) {
+            float e = 0.0f;
+            if (hipEventElapsedTime(&e, p.first, p.second) == hipSuccess) t += e;
+        }
+        if (ms) *ms = t;
+    };
+    total(s->prof_step, step_ms);
+    total(s->prof_rebuild, rebuild_ms);
+    if (step_launches) *step_launches = (int64_t)s->prof_step.size();
+    if (rebuilds) *rebuilds = (int64_t)s->prof_rebuild.size();
+    if (reset) {
+        s->prof_step.clear();
+        s->prof_rebuild.clear();
+        s->ev_used = 0;
+    }
+    return PARM_OK;
+}
+
+int parm_synchronize(parm_system *s) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(set_device(s));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    return PARM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+// common.hpp -- shared device helpers for the ParM MD hot path on gfx950.
+//
+// Everything here is compiled with -ffp-contract=off: an expression rounds as
+// written unless it calls fma() explicitly.  The integrators rely on that to
+// reproduce the reference's double-precision update bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace parm {
+
+constexpr int kBlock = 256;   // threads per workgroup for per-atom kernels
+constexpr int kWave = 64;     // CDNA wavefront
+
+// ---------------------------------------------------------------- vectors
+template <int D>
+struct Vec {
+    double c[D];
+};
+
+// Position storage: 3D positions live in a double4 (one 32-byte load per
+// neighbour gather, the 4th lane unused); 2D positions in a double2.
+template <int D>
+struct PosStore;
+template <>
+struct PosStore<3> {
+    using T = double4;
+    __host__ __device__ static inline Vec<3> load(const T &p) {
+        return Vec<3>{{p.x, p.y, p.z}};
+    }
+    __host__ __device__ static inline T make(const Vec<3> &v) {
+        return T{v.c[0], v.c[1], v.c[2], 0.0};
+    }
+};
+template <>
+struct PosStore<2> {
+    using T = double2;
+    __host__ __device__ static inline Vec<2> load(const T &p) {
+        return Vec<2>{{p.x, p.y}};
+    }
+    __host__ __device__ static inline T make(const Vec<2> &v) {
+        return T{v.c[0], v.c[1]};
+    }
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Renumbering them so that XCD k runs one contiguous range of logical
+// workgroups keeps spatially adjacent atoms (adjacent slots) on one L2, so
+// the neighbour positions they share are fetched into one L2, not eight.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int x = b % kXcds, k = b / kXcds;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// Box: edge lengths, their halves and inverses.
+struct Box {
+    double L[3];
+    double invL[3];
+    double halfL[3];
+};
+
+// |r|^2 summed as (x*x + y*y) + z*z, the order the reference's Eigen
+// Vec::squaredNorm uses for a 3-vector of doubles.
+template <int D>
+__device__ __forceinline__ double sqnorm(const Vec<D> &r) {
+    if constexpr (D == 3)
+        return (r.c[0] * r.c[0] + r.c[1] * r.c[1]) + r.c[2] * r.c[2];
+    else
+        return r.c[0] * r.c[0] + r.c[1] * r.c[1];
+}
+
+// Minimum image of one component, equal to IEEE remainder(dx, L) bit for bit
+// (OriginBox::diff, src/box.hpp:68-78,103).  n = rint(dx * (1/L)) is a guess
+// at the nearest integer to dx/L.  r = fma(-n, L, dx) is dx - n L rounded
+// once.  Rounding is monotone and L/2 is representable, so |r| < L/2 implies
+// the exact dx - n L is below L/2 too: n is then the nearest integer, and the
+// exact result (remainder's, always representable) is what fma returned.
+// Otherwise (|r| >= L/2: a wrong guess or a tie) the library remainder runs.
+__device__ __forceinline__ double min_image1(double dx, double L, double invL,
+                                             double halfL) {
+    double n = rint(dx * invL);
+    double r = fma(-n, L, dx);
+    if (__builtin_expect(!(fabs(r) < halfL), 0)) r = remainder(dx, L);
+    return r;
+}
+
+template <int D>
+__device__ __forceinline__ Vec<D> diff(const Vec<D> &a, const Vec<D> &b,
+                                       const Box &box) {
+    Vec<D> r;
+#pragma unroll
+    for (int k = 0; k < D; k++)
+        r.c[k] = min_image1(a.c[k] - b.c[k], box.L[k], box.invL[k],
+                            box.halfL[k]);
+    return r;
+}
+
+// 1/x to ~1 ulp: hardware estimate and two Newton-Raphson steps.
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    return r;
+}
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Neumaier-compensated running sum (value + compensation).
+struct KSum {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x) {
+        double t = s + x;
+        if (fabs(s) >= fabs(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+
+// Compensated pair (sum, error) combined with Knuth's two-sum: used to merge
+// Neumaier partial sums in a fixed tree.
+struct CSum {
+    double s = 0.0, c = 0.0;
+};
+__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
+    const double t = a.s + b.s;
+    const double bp = t - a.s;
+    const double err = (a.s - (t - bp)) + (b.s - bp);
+    return CSum{t, a.c + b.c + err};
+}
+
+// Top-2 of non-negative values (ties kept: {d, d} has sum 2d).
+struct Top2 {
+    double a = 0.0, b = 0.0;  // a >= b
+    __device__ __forceinline__ void add(double d) {
+        if (d > a) {
+            b = a;
+            a = d;
+        } else if (d > b) {
+            b = d;
+        }
+    }
+    __device__ __forceinline__ void merge(const Top2 &o) {
+        add(o.a);
+        add(o.b);
+    }
+};
+
+__device__ __forceinline__ Top2 wave_top2(Top2 t) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        Top2 o;
+        o.a = __shfl_xor(t.a, off);
+        o.b = __shfl_xor(t.b, off);
+        t.merge(o);
+    }
+    return t;
+}
+
+// Flag word written by kernels and copied to pinned host memory.
+enum : uint32_t {
+    kFlagRebuild = 1u,      // skin criterion met: rebuild before the next forces
+    kFlagNonFinite = 2u,    // a displacement was NaN/Inf
+    kFlagOverflow = 4u,     // neighbour capacity exceeded during a build
+};
+
+}  // namespace parm

@@ -1,0 +1,140 @@
+// kernels.hpp -- host-side launchers of the device kernels (internal API of
+// libparm_hip.so; the public C ABI is include/parm_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace parm {
+
+// Periodic cell grid used by the list rebuild.
+struct CellGrid {
+    int nc[3];       // cells per dimension (1 when the box is under 3 list radii)
+    int ncell;       // product of nc over the active dimensions
+    double invL[3];  // 1 / box edge
+};
+
+// ---------------------------------------------------------------- nlist.hip
+int radix_blocks(int n);
+hipError_t launch_cell_keys(int ndim, int n, const void *pos, const int32_t *rank,
+                            const CellGrid &g, uint32_t *keys, int32_t *vals,
+                            hipStream_t s);
+hipError_t radix_sort(int n, int bits, uint32_t *keys, int32_t *vals,
+                      uint32_t *tmp_keys, int32_t *tmp_vals, uint32_t *hist,
+                      hipStream_t s);
+template <typename T>
+hipError_t gather(int n, const int32_t *perm, const T *src, T *dst, hipStream_t s);
+hipError_t launch_cell_bounds(int n, int ncell, const uint32_t *keys, int32_t *start,
+                              hipStream_t s);
+// Builds the full list (and xref = pos); stats[0] = sum of counts, stats[1] =
+// max count; part holds 2 * build_blocks(n) scratch words.
+int build_blocks(int n);
+hipError_t launch_build_list(int ndim, int n, const void *pos, const int32_t *rank,
+                             const double *diam, const int32_t *type,
+                             const uint32_t *keys, const int32_t *cstart,
+                             const CellGrid &g, const Box &box, double skin,
+                             int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
+                             int32_t *ncount, void *xref, unsigned long long *part,
+                             unsigned long long *stats, uint32_t *flag,
+                             hipStream_t s);
+
+// -------------------------------------------------------------- step.hip
+enum PairKind : int { kPairNone = 0, kPairLJ = 1, kPairRepulsion = 2 };
+enum Precision : int { kFP64 = 0, kMixed = 1 };
+
+// Per type-pair parameters, precomputed on the host exactly as the
+// reference's pair constructors compute them (src/interaction.hpp:1256-1269,
+// 1532-1537).  Indexed [t1 * ntypes + t2].
+struct PairParam {
+    double eps;        // LJ: |eps| ; Repulsion: sqrt(e1 e2)
+    double sig;        // (s1 + s2) / 2
+    double sig2;       // sig * sig
+    double cut2;       // LJ: cut * cut (in sigma units) ; Repulsion: sig * sig
+    double lim_lo;     // d2 below: surely inside the cutoff
+    double lim_hi;     // d2 above: surely outside
+    double p5;         // LJ: cut_energy      ; Repulsion: exponent n
+    double p6;         // LJ: 12 * eps        ; Repulsion: n - 1
+    float eps_f, sig2_f, p5_f, p6_f;  // single-precision copies (mixed mode)
+    float lim_f, sig_f, isig_f, pad_f;
+};
+
+// Per-system views handed to the step kernels.
+struct AtomArrays {
+    int n;                 // slots
+    const void *pos_in;    // PosStore<D>::T*
+    void *pos_out;         // fused steps write the drifted positions here
+    double *v[3], *a[3], *f[3];
+    const double *m;
+    const int32_t *type;   // pair-table type of each slot
+    const int32_t *rank;   // < 0: not a list member (no pair forces, no skin test)
+    const void *xref;      // positions at the last rebuild
+};
+
+struct ListView {
+    const uint32_t *nbr;
+    const int32_t *ncount;
+    int npad;
+    int type_bits;
+    int tpa;               // lanes per atom; entries grouped by tpa (see k_force)
+    int built;             // 0: the list was never built -> no pairs
+};
+
+struct PairView {
+    int kind;
+    int ntypes;
+    int single;            // 1: one type pair (parameters uniform)
+    int harmonic;          // Repulsion with n == 2 for every type pair
+    const PairParam *tab;
+};
+
+// Integrator stage fused into the force kernel.
+enum Stage : int {
+    kStageForceOnly = 0,   // Collection::set_forces(false): f only
+    kStageForceA = 1,      // set_forces(true): f and a = f/m
+    kStageVerletKick = 2,  // Verlet second half: a = f/m ; v += a dt/2
+    kStageVerletKickDrift = 3,  // ... then the next step's drift + half kick
+    kStageDampedKick = 4,  // Damped: a = f/m ; v += a dt c2
+    kStageDampedKickDrift = 5,
+    kStageSums = 6,        // energy / virial / stress only, nothing written
+};
+
+struct StepCoef {
+    // Verlet: dt, dt*dt/2, dt/2.  Damped: c1*dt, c2*dt*dt, dt*(c1-c2), c0, dt*c2
+    double k0, k1, k2, k3, k4;
+};
+
+// Reduction scratch: one partial per workgroup, reduced by k_finalize.
+struct ReduceBuf {
+    double *part;                 // nblocks * kRedWidth doubles
+    double *out;                  // kRedWidth doubles (final sums)
+    uint32_t *flag;               // flag word
+    double skin;                  // skin test threshold
+};
+constexpr int kRedWidth = 16;     // U, W, S[9], contacts, overlaps, top1, top2, nonfinite
+
+hipError_t launch_force(int ndim, int precision, int stage, bool energy,
+                        const AtomArrays &A, const ListView &Lv, const PairView &P,
+                        const Box &box, const StepCoef &c, const ReduceBuf &R,
+                        hipStream_t s);
+hipError_t launch_drift(int ndim, int stage, const AtomArrays &A, const StepCoef &c,
+                        const ReduceBuf &R, hipStream_t s);
+hipError_t launch_skin(int ndim, const AtomArrays &A, const ReduceBuf &R,
+                       hipStream_t s);
+// Atom observables: mode 0 -> out[0] = K(originv), out[1..D] = momentum,
+// out[4] = mass ; mode 1 -> out[0] = sum |f|, out[13] = max |f|.
+hipError_t launch_atom_sums(int ndim, int mode, const AtomArrays &A,
+                            const double *originv, const ReduceBuf &R, hipStream_t s);
+// Layout conversion between the caller's AoS order and the device slots.
+// upload: dst component k of slot s = src[orig[s]*ndim + k]  (what: 0 pos,
+// 1 v, 2 a, 3 f);  download: the inverse.
+hipError_t launch_upload(int ndim, int what, const AtomArrays &A, const int32_t *orig,
+                         const double *src, hipStream_t s);
+hipError_t launch_download(int ndim, int what, const AtomArrays &A,
+                           const int32_t *orig, double *dst, hipStream_t s);
+int reduce_blocks(int n);
+int force_blocks(int n, int tpa);  // workgroups of a force launch (also partials)
+
+}  // namespace parm

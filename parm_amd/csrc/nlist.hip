@@ -1,0 +1,463 @@
+// nlist.hip -- device Verlet-list rebuild for NeighborList::update_list
+// (reference: src/trackers.cpp:19-85).
+//
+// The reference rebuild is an O(N^2) double loop over members testing
+//   |diff(x_i, x_j)| < (d_i + d_j)/2 + skin.
+// Here the same predicate, evaluated bit-identically, runs over a periodic
+// cell grid (cell edge >= the largest list radius) after a stable LSD radix
+// sort of the atoms by cell.  The sort also renumbers the device slots, so
+// atoms that are close in space are close in memory for the force gathers.
+//
+// Pipeline (one rebuild):
+//   k_cell_keys      key = cell of each member, ncell for non-members
+//   radix passes     k_radix_hist -> k_radix_scan -> k_radix_scatter (8 bits)
+//   gathers          every per-slot array permuted into the new order
+//   k_cell_bounds    first slot of each cell
+//   k_build_list<D>  full (both directions) list, ELL layout nbr[k*npad + i]
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace parm {
+
+// ------------------------------------------------------------ cell keys
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_cell_keys(
+    int n, const typename PosStore<D>::T *__restrict__ pos,
+    const int32_t *__restrict__ rank, CellGrid g, uint32_t *__restrict__ keys,
+    int32_t *__restrict__ vals) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    vals[i] = i;
+    if (rank[i] < 0) {
+        keys[i] = (uint32_t)g.ncell;
+        return;
+    }
+    Vec<D> x = PosStore<D>::load(pos[i]);
+    uint32_t key = 0;
+#pragma unroll
+    for (int k = D - 1; k >= 0; k--) {
+        double s = x.c[k] * g.invL[k];
+        s -= floor(s);
+        int c = (int)(s * (double)g.nc[k]);
+        c = c < 0 ? 0 : (c >= g.nc[k] ? g.nc[k] - 1 : c);
+        key = key * (uint32_t)g.nc[k] + (uint32_t)c;
+    }
+    keys[i] = key;
+}
+
+// ------------------------------------------------------------ radix sort
+// Stable LSD radix sort of (key, value) pairs, 8 bits per pass.  Each block
+// owns a tile of kRadixTile consecutive items.  hist[d * nblocks + b] counts
+// digit d in tile b; its exclusive scan is the output offset of (d, b), which
+// keeps equal digits in input order across tiles.  Within a tile the scatter
+// ranks equal digits in input order with 64-lane ballots.
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kBlock * kRadixItems;
+
+__global__ __launch_bounds__(kBlock) void k_radix_hist(
+    int n, int shift, const uint32_t *__restrict__ keys,
+    uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * kRadixTile;
+    for (int t = 0; t < kRadixItems; t++) {
+        int i = base + t * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of m entries (m a multiple of 4) by one workgroup of 1024
+// threads: each thread owns a contiguous chunk (a multiple of 4 entries) that
+// it reads and writes with 16-byte accesses, so its loads are in flight
+// together; the chunk totals are scanned in LDS.
+__global__ __launch_bounds__(1024) void k_radix_scan(int m,
+                                                     uint32_t *__restrict__ a) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = ((m + 1023) / 1024 + 3) / 4 * 4;
+    const int lo = min(m, t * per), hi = min(m, lo + per);
+    uint4 *a4 = reinterpret_cast<uint4 *>(a);
+    uint32_t s = 0;
+    for (int i = lo; i < hi; i += 4) {
+        const uint4 v = a4[i / 4];
+        s += v.x + v.y + v.z + v.w;
+    }
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
+    for (int i = lo; i < hi; i += 4) {
+        uint4 v = a4[i / 4];
+        uint4 o;
+        o.x = run;
+        run += v.x;
+        o.y = run;
+        run += v.y;
+        o.z = run;
+        run += v.z;
+        o.w = run;
+        run += v.w;
+        a4[i / 4] = o;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(
+    int n, int shift, const uint32_t *__restrict__ keys_in,
+    const int32_t *__restrict__ vals_in, const uint32_t *__restrict__ offs,
+    uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wcnt[kBlock / kWave][256];
+    const int t = threadIdx.x;
+    const int w = t / kWave;
+    const int lane = t % kWave;
+    base[t] = offs[t * gridDim.x + blockIdx.x];
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int tile = blockIdx.x * kRadixTile;
+    for (int c = 0; c < kRadixItems; c++) {
+#pragma unroll
+        for (int q = 0; q < kBlock / kWave; q++) wcnt[q][t] = 0;
+        __syncthreads();
+        int i = tile + c * kBlock + t;
+        bool valid = i < n;
+        uint32_t key = valid ? keys_in[i] : 0u;
+        uint32_t d = (key >> shift) & 255u;
+        unsigned long long eq = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            unsigned long long bb = __ballot(bit);
+            eq &= bit ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(eq & lt);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(eq);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = base[d];
+            for (int q = 0; q < w; q++) off += wcnt[q][d];
+            off += rank;
+            keys_out[off] = key;
+            vals_out[off] = vals_in[i];
+        }
+        __syncthreads();
+        // advance the per-digit base past this chunk (thread t owns digit t)
+        uint32_t tot = 0;
+#pragma unroll
+        for (int q = 0; q < kBlock / kWave; q++) tot += wcnt[q][t];
+        base[t] += tot;
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- gathers
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather(int n, const int32_t *__restrict__ perm,
+                                                   const T *__restrict__ src,
+                                                   T *__restrict__ dst) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+// ---------------------------------------------------------- cell bounds
+__global__ __launch_bounds__(kBlock) void k_cell_bounds(
+    int n, int ncell, const uint32_t *__restrict__ keys,
+    int32_t *__restrict__ start) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i > n) return;
+    long prev = (i == 0) ? -1 : (long)keys[i - 1];
+    long cur = (i == n) ? (long)ncell : (long)keys[i];
+    if (i == n) {
+        for (long c = prev + 1; c <= ncell; c++) start[c] = n;
+        return;
+    }
+    for (long c = prev + 1; c <= cur && c <= ncell; c++) start[c] = i;
+}
+
+// ----------------------------------------------------------- list build
+// G lanes per slot.  The candidates of slot i are the slots of its stencil,
+// taken as rows of three x-adjacent cells, which are contiguous slot ranges
+// (two ranges where the row wraps around the box).  In each round the G lanes
+// test G consecutive candidates and the passing ones are appended in
+// candidate order (ballot + prefix popcount), so the list order is
+// deterministic.  Entry q of slot i goes to ((q / tpa) * npad + i) * tpa +
+// q % tpa: groups of tpa consecutive entries, the layout k_force reads.
+//
+// The predicate is the reference's, bit for bit:
+//   r = diff(x_i, x_j) (remainder per component), d2 = (x^2 + y^2) + z^2,
+//   in list  <=>  sqrt(d2) < (d_i + d_j)/2 + skin   (correctly rounded sqrt).
+// sqrt is only evaluated when d2 is within 1e-14 (relative) of c^2; outside
+// that band the comparison with c^2 decides it with margin to spare.
+template <int D>
+__device__ __forceinline__ bool in_list(const Vec<D> &xi, const Vec<D> &xj, double di,
+                                        double dj, const Box &box, double skin) {
+    const Vec<D> r = diff<D>(xi, xj, box);
+    const double d2 = sqnorm<D>(r);
+    const double c = (di + dj) / 2 + skin;
+    const double c2 = c * c;
+    if (d2 < c2 * (1.0 - 1e-14)) return true;
+    if (d2 > c2 * (1.0 + 1e-14)) return false;
+    return sqrt(d2) < c;
+}
+
+template <int D, int G>
+__global__ __launch_bounds__(kBlock) void k_build_list(
+    int n, const typename PosStore<D>::T *__restrict__ pos,
+    const int32_t *__restrict__ rank, const double *__restrict__ diam,
+    const int32_t *__restrict__ type, const uint32_t *__restrict__ keys,
+    const int32_t *__restrict__ cstart, CellGrid g, Box box, double skin,
+    int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
+    int32_t *__restrict__ ncount, typename PosStore<D>::T *__restrict__ xref,
+    unsigned long long *__restrict__ part /* per workgroup: sum, max */,
+    uint32_t *__restrict__ flag) {
+    __shared__ unsigned long long red[kBlock / kWave][2];
+    const int gt = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int i = gt / G;
+    const int sub = gt % G;
+    const int lane = threadIdx.x % kWave;
+    const int gbase = lane - sub;  // first lane of this slot's group
+    const unsigned long long below = (sub == 0) ? 0ull : ((1ull << sub) - 1ull);
+    unsigned long long mycnt = 0;
+    if (i < n) {
+        const typename PosStore<D>::T pi = pos[i];
+        if (sub == 0) xref[i] = pi;
+        if (rank[i] < 0) {
+            if (sub == 0) ncount[i] = 0;
+        } else {
+            const Vec<D> xi = PosStore<D>::load(pi);
+            const double di = diam[i];
+            uint32_t key = keys[i];
+            int ci[3] = {0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                ci[k] = (int)(key % (uint32_t)g.nc[k]);
+                key /= (uint32_t)g.nc[k];
+            }
+            const bool wide0 = g.nc[0] >= 3;
+            const int oyl = (D >= 2 && g.nc[1] >= 3) ? -1 : 0;
+            const int ozl = (D == 3 && g.nc[2] >= 3) ? -1 : 0;
+            int cnt = 0;
+            for (int oz = ozl; oz <= -ozl; oz++) {
+                for (int oy = oyl; oy <= -oyl; oy++) {
+                    int q1 = ci[1] + oy, q2 = ci[2] + oz;
+                    if (D >= 2) q1 = q1 < 0 ? q1 + g.nc[1] : (q1 >= g.nc[1] ? q1 - g.nc[1] : q1);
+                    if (D == 3) q2 = q2 < 0 ? q2 + g.nc[2] : (q2 >= g.nc[2] ? q2 - g.nc[2] : q2);
+                    const int rowc = (D == 3 ? q2 * g.nc[1] + q1 : q1) * g.nc[0];
+                    // the row's x cells as at most two contiguous cell ranges
+                    int c0[2], c1[2], nr = 1;
+                    if (!wide0) {
+                        c0[0] = 0;
+                        c1[0] = 0;
+                    } else if (ci[0] == 0) {
+                        c0[0] = 0;
+                        c1[0] = 1;
+                        c0[1] = g.nc[0] - 1;
+                        c1[1] = g.nc[0] - 1;
+                        nr = 2;
+                    } else if (ci[0] == g.nc[0] - 1) {
+                        c0[0] = ci[0] - 1;
+                        c1[0] = ci[0];
+                        c0[1] = 0;
+                        c1[1] = 0;
+                        nr = 2;
+                    } else {
+                        c0[0] = ci[0] - 1;
+                        c1[0] = ci[0] + 1;
+                    }
+                    for (int rr = 0; rr < nr; rr++) {
+                        const int a = cstart[rowc + c0[rr]];
+                        const int b = cstart[rowc + c1[rr] + 1];
+                        for (int base = a; base < b; base += G) {
+                            const int j = base + sub;
+                            bool pass = false;
+                            if (j < b && j != i)
+                                pass = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
+                                                  diam[j], box, skin);
+                            const unsigned long long m =
+                                (__ballot(pass) >> gbase) & ((G == 64) ? ~0ull : ((1ull << G) - 1ull));
+                            if (pass) {
+                                const int q = cnt + __popcll(m & below);
+                                if (q < cap)
+                                    nbr[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] =
+                                        ((uint32_t)j << type_bits) | (uint32_t)type[j];
+                            }
+                            cnt += __popcll(m);
+                        }
+                    }
+                }
+            }
+            if (sub == 0) {
+                ncount[i] = cnt < cap ? cnt : cap;
+                if (cnt > cap) atomicOr(flag, kFlagOverflow);
+                mycnt = (unsigned long long)cnt;
+            }
+        }
+    }
+    // per-workgroup sum and max of the counts (no same-address atomics)
+    unsigned long long sum = mycnt, mx = mycnt;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off);
+        const unsigned long long o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if (lane == 0) {
+        red[threadIdx.x / kWave][0] = sum;
+        red[threadIdx.x / kWave][1] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s2 = 0, m2 = 0;
+        for (int q = 0; q < kBlock / kWave; q++) {
+            s2 += red[q][0];
+            m2 = red[q][1] > m2 ? red[q][1] : m2;
+        }
+        part[2 * (size_t)blockIdx.x] = s2;
+        part[2 * (size_t)blockIdx.x + 1] = m2;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_stats(
+    int nparts, const unsigned long long *__restrict__ part,
+    unsigned long long *__restrict__ stats) {
+    __shared__ unsigned long long red[kBlock / kWave][2];
+    unsigned long long sum = 0, mx = 0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
+        sum += part[2 * (size_t)b];
+        mx = part[2 * (size_t)b + 1] > mx ? part[2 * (size_t)b + 1] : mx;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off);
+        const unsigned long long o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    const int lane = threadIdx.x % kWave;
+    if (lane == 0) {
+        red[threadIdx.x / kWave][0] = sum;
+        red[threadIdx.x / kWave][1] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s2 = 0, m2 = 0;
+        for (int q = 0; q < kBlock / kWave; q++) {
+            s2 += red[q][0];
+            m2 = red[q][1] > m2 ? red[q][1] : m2;
+        }
+        stats[0] = s2;
+        stats[1] = m2;
+    }
+}
+
+// ======================================================== host launchers
+static inline int nblk(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+hipError_t launch_cell_keys(int ndim, int n, const void *pos, const int32_t *rank,
+                            const CellGrid &g, uint32_t *keys, int32_t *vals,
+                            hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_cell_keys<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
+                           (const double4 *)pos, rank, g, keys, vals);
+    else
+        hipLaunchKernelGGL(k_cell_keys<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
+                           (const double2 *)pos, rank, g, keys, vals);
+    return hipGetLastError();
+}
+
+int radix_blocks(int n) { return (n + kRadixTile - 1) / kRadixTile; }
+
+// Sorts (keys, vals) by the low `bits` bits of the key; the result ends in
+// (keys, vals).  tmp_keys/tmp_vals are scratch of n entries; hist holds
+// 256 * radix_blocks(n) counters.
+hipError_t radix_sort(int n, int bits, uint32_t *keys, int32_t *vals,
+                      uint32_t *tmp_keys, int32_t *tmp_vals, uint32_t *hist,
+                      hipStream_t s) {
+    if (n <= 1) return hipSuccess;
+    const int nb = radix_blocks(n);
+    uint32_t *ka = keys, *kb = tmp_keys;
+    int32_t *va = vals, *vb = tmp_vals;
+    int passes = (bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    for (int p = 0; p < passes; p++) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, s, n, shift,
+                           ka, hist);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, 256 * nb, hist);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, s, n, shift,
+                           ka, va, hist, kb, vb);
+        uint32_t *tk = ka;
+        ka = kb;
+        kb = tk;
+        int32_t *tv = va;
+        va = vb;
+        vb = tv;
+    }
+    if (ka != keys) {
+        hipError_t e = hipMemcpyAsync(keys, ka, sizeof(uint32_t) * n,
+                                      hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        e = hipMemcpyAsync(vals, va, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t gather(int n, const int32_t *perm, const T *src, T *dst, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather<T>, dim3(nblk(n)), dim3(kBlock), 0, s, n, perm, src,
+                       dst);
+    return hipGetLastError();
+}
+template hipError_t gather<double>(int, const int32_t *, const double *, double *,
+                                   hipStream_t);
+template hipError_t gather<int32_t>(int, const int32_t *, const int32_t *, int32_t *,
+                                    hipStream_t);
+template hipError_t gather<double2>(int, const int32_t *, const double2 *, double2 *,
+                                    hipStream_t);
+template hipError_t gather<double4>(int, const int32_t *, const double4 *, double4 *,
+                                    hipStream_t);
+
+hipError_t launch_cell_bounds(int n, int ncell, const uint32_t *keys, int32_t *start,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_bounds, dim3(nblk(n + 1)), dim3(kBlock), 0, s, n,
+                       ncell, keys, start);
+    return hipGetLastError();
+}
+
+constexpr int kBuildLanes = 8;  // lanes per slot in k_build_list
+
+int build_blocks(int n) { return (int)(((long)n * kBuildLanes + kBlock - 1) / kBlock); }
+
+hipError_t launch_build_list(int ndim, int n, const void *pos, const int32_t *rank,
+                             const double *diam, const int32_t *type,
+                             const uint32_t *keys, const int32_t *cstart,
+                             const CellGrid &g, const Box &box, double skin,
+                             int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
+                             int32_t *ncount, void *xref, unsigned long long *part,
+                             unsigned long long *stats, uint32_t *flag,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int nb = build_blocks(n);
+    if (ndim == 3)
+        hipLaunchKernelGGL((k_build_list<3, kBuildLanes>), dim3(nb), dim3(kBlock), 0, s,
+                           n, (const double4 *)pos, rank, diam, type, keys, cstart, g,
+                           box, skin, type_bits, tpa, cap, npad, nbr, ncount,
+                           (double4 *)xref, part, flag);
+    else
+        hipLaunchKernelGGL((k_build_list<2, kBuildLanes>), dim3(nb), dim3(kBlock), 0, s,
+                           n, (const double2 *)pos, rank, diam, type, keys, cstart, g,
+                           box, skin, type_bits, tpa, cap, npad, nbr, ncount,
+                           (double2 *)xref, part, flag);
+    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, nb, part, stats);
+    return hipGetLastError();
+}
+
+}  // namespace parm

@@ -1,0 +1,840 @@
+// step.hip -- pair forces over the device Verlet list, fused with the
+// velocity-Verlet / damped-Verlet update and the skin test.
+//
+// Reference behaviour restated (paths relative to the reference root):
+//   NListed<A,P>::set_forces / energy / pressure / stress
+//                                   src/interaction.hpp:2153-2291
+//   LJAttractRepulsePair::forces/energy  src/interaction.hpp:1270-1298
+//   RepulsionPair::forces/energy         src/interaction.hpp:1538-1556
+//   Collection::set_forces               src/collection.cpp:159-179
+//   CollectionVerlet::timestep           src/collection.cpp:442-469
+//   CollectionDamped::timestep           src/collection.cpp:356-381
+//   NeighborList::update_list skin test  src/trackers.cpp:23-53
+//
+// The device list is a FULL list (each pair appears in both atoms' rows), so
+// each thread gathers the force on its own atom without atomics and in a
+// fixed order.  The per-pair force from j on i uses r = diff(x_i, x_j); the
+// visit from j computes -r bit for bit, so the two visits are exact negatives
+// and only the summation order differs from the reference's half-list
+// scatter.  Energies, virials and contact counts are halved.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace parm {
+
+// ------------------------------------------------------ reductions
+// A kernel that reduces (sums of energy / virial / stress / counts, and the
+// skin test's top-2 and non-finite count) writes one partial per workgroup
+// with plain stores; k_finalize, launched right after it on the same stream,
+// reduces the partials in a fixed order.  No fences or contended atomics are
+// needed inside the hot kernel, and the result does not depend on scheduling.
+constexpr int kSumU = 0, kSumW = 1, kSumS = 2, kSumContacts = 11,
+              kSumOverlaps = 12, kTop1 = 13, kTop2 = 14, kNonFinite = 15;
+
+template <bool SUMS, bool SKIN>
+__device__ __forceinline__ void block_partial(double (&v)[kRedWidth], Top2 t,
+                                              const ReduceBuf &R) {
+    __shared__ double sh[kBlock / kWave][kRedWidth];
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (SUMS) {
+#pragma unroll
+        for (int k = 0; k < kTop1; k++) v[k] = wave_sum(v[k]);
+    }
+    if (SKIN) {
+        t = wave_top2(t);
+        v[kNonFinite] = wave_sum(v[kNonFinite]);
+    }
+    if (lane == 0) {
+        if (SUMS) {
+#pragma unroll
+            for (int k = 0; k < kTop1; k++) sh[w][k] = v[k];
+        }
+        if (SKIN) {
+            sh[w][kTop1] = t.a;
+            sh[w][kTop2] = t.b;
+            sh[w][kNonFinite] = v[kNonFinite];
+        }
+    }
+    __syncthreads();
+    const int k = threadIdx.x;
+    double *dst = R.part + (size_t)blockIdx.x * kRedWidth;
+    if (SUMS && k < kTop1) {
+        double p = 0.0;
+        for (int q = 0; q < kBlock / kWave; q++) p += sh[q][k];
+        dst[k] = p;
+    }
+    if (SKIN && k == kTop1) {
+        Top2 bt;
+        double nf = 0.0;
+        for (int q = 0; q < kBlock / kWave; q++) {
+            Top2 o;
+            o.a = sh[q][kTop1];
+            o.b = sh[q][kTop2];
+            bt.merge(o);
+            nf += sh[q][kNonFinite];
+        }
+        dst[kTop1] = bt.a;
+        dst[kTop2] = bt.b;
+        dst[kNonFinite] = nf;
+    }
+}
+
+// One workgroup: thread t reduces partials t, t + kBlock, ... (Neumaier),
+// then a fixed butterfly and a fixed 4-wave combine (compensated two-sum).
+template <bool SUMS, bool SKIN>
+__global__ __launch_bounds__(kBlock) void k_finalize(int nparts, ReduceBuf R) {
+    const int tid = threadIdx.x;
+    const int lane = tid % kWave, w = tid / kWave;
+    if constexpr (SUMS) {
+        CSum cs[kTop1];
+        for (int b = tid; b < nparts; b += kBlock) {
+            const double *src = R.part + (size_t)b * kRedWidth;
+#pragma unroll
+            for (int k = 0; k < kTop1; k++) {
+                KSum q{cs[k].s, cs[k].c};
+                q.add(src[k]);
+                cs[k] = CSum{q.s, q.c};
+            }
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < kTop1; k++) {
+                CSum o{__shfl_xor(cs[k].s, off), __shfl_xor(cs[k].c, off)};
+                cs[k] = csum_merge(cs[k], o);
+            }
+        }
+        __shared__ double fin[kBlock / kWave][2 * kTop1];
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < kTop1; k++) {
+                fin[w][2 * k] = cs[k].s;
+                fin[w][2 * k + 1] = cs[k].c;
+            }
+        }
+        __syncthreads();
+        if (tid < kTop1) {
+            CSum tot;
+            for (int q = 0; q < kBlock / kWave; q++)
+                tot = csum_merge(tot, CSum{fin[q][2 * tid], fin[q][2 * tid + 1]});
+            R.out[tid] = tot.s + tot.c;
+        }
+    }
+    if constexpr (SKIN) {
+        Top2 bt;
+        CSum nf;
+        for (int b = tid; b < nparts; b += kBlock) {
+            const double *src = R.part + (size_t)b * kRedWidth;
+            Top2 o;
+            o.a = src[kTop1];
+            o.b = src[kTop2];
+            bt.merge(o);
+            KSum q{nf.s, nf.c};
+            q.add(src[kNonFinite]);
+            nf = CSum{q.s, q.c};
+        }
+        bt = wave_top2(bt);
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1)
+            nf = csum_merge(nf, CSum{__shfl_xor(nf.s, off), __shfl_xor(nf.c, off)});
+        __shared__ double fin2[kBlock / kWave][3];
+        if (lane == 0) {
+            fin2[w][0] = bt.a;
+            fin2[w][1] = bt.b;
+            fin2[w][2] = nf.s + nf.c;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            Top2 tt;
+            double nft = 0.0;
+            for (int q = 0; q < kBlock / kWave; q++) {
+                Top2 o;
+                o.a = fin2[q][0];
+                o.b = fin2[q][1];
+                tt.merge(o);
+                nft += fin2[q][2];
+            }
+            R.out[kTop1] = tt.a;
+            R.out[kTop2] = tt.b;
+            R.out[kNonFinite] = nft;
+            uint32_t fl = 0;
+            if (nft > 0) fl |= kFlagNonFinite;
+            // NeighborList::update_list (src/trackers.cpp:23-53) checks
+            // bigdist + biggestdist >= skin only after one of the two grew,
+            // which needs a displacement > 0: when nothing moved it never
+            // rebuilds.
+            if (tt.a > 0 && tt.b + tt.a >= R.skin) fl |= kFlagRebuild;
+            *R.flag |= fl;
+        }
+    }
+}
+
+template <bool SUMS, bool SKIN>
+static void launch_finalize(int nparts, const ReduceBuf &R, hipStream_t s) {
+    hipLaunchKernelGGL((k_finalize<SUMS, SKIN>), dim3(1), dim3(kBlock), 0, s, nparts, R);
+}
+
+// ------------------------------------------------------------ pair math
+struct PairAcc {
+    double e = 0.0, w = 0.0, s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double contacts = 0.0, overlaps = 0.0;
+};
+
+// FP64 LJAttractRepulsePair on one visit; returns false if outside cutoff.
+// Cutoff: the reference skips when d2 / sig^2 > cut^2; lim_lo / lim_hi bracket
+// that boundary so the division only runs within 1e-14 of it.
+template <int D, bool ENERGY>
+__device__ __forceinline__ void lj64(const PairParam &pp, const Vec<D> &r,
+                                     double (&f)[D], PairAcc &acc) {
+    const double d2 = sqnorm<D>(r);
+    bool in = d2 < pp.lim_lo ||
+              (!(d2 > pp.lim_hi) && !(d2 / pp.sig2 > pp.cut2));
+    if (!in) return;
+    const double idsq = rcp64(d2);
+    const double ir = pp.sig2 * idsq;         // sigma^2 / r^2
+    const double rsix = ir * ir * ir;         // sigma^6 / r^6
+    const double fmagR = pp.p6 * rsix * (rsix - 1.0);  // 12 eps s (s - 1)
+    const double fs = fmagR * idsq;
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = fma(fs, r.c[k], f[k]);
+    if (ENERGY) {
+        const double mid = 1.0 - rsix;
+        const double e = pp.eps * (mid * mid) - pp.p5;
+        acc.e += e;
+        acc.w += fmagR;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int b = 0; b < D; b++) acc.s[a * D + b] += r.c[a] * (r.c[b] * fs);
+        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
+        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
+    }
+}
+
+// FP64 RepulsionPair: same operations as the reference, so each pair force
+// is bit-identical to the reference's for n == 2 (pow(x,1) = x, pow(x,2) = x*x).
+template <int D, bool ENERGY, bool HARM>
+__device__ __forceinline__ void rep64(const PairParam &pp, const Vec<D> &r,
+                                      double (&f)[D], PairAcc &acc) {
+    const double d2 = sqnorm<D>(r);
+    if (d2 > pp.sig2) return;
+    const double R = sqrt(d2);
+    const double x = 1.0 - (R / pp.sig);
+    const double px = HARM ? x : pow(x, pp.p6);
+    const double fs = pp.eps * px / pp.sig / R;
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = fma(r.c[k], fs, f[k]);
+    if (ENERGY) {
+        const double e = pp.eps * (HARM ? x * x : pow(x, pp.p5)) / pp.p5;
+        acc.e += e;
+        double rf = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) rf += r.c[k] * (r.c[k] * fs);
+        acc.w += rf;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int b = 0; b < D; b++) acc.s[a * D + b] += r.c[a] * (r.c[b] * fs);
+        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
+        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
+    }
+}
+
+// Mixed precision: the displacement is formed in FP64 (exact minimum image),
+// then r^2, r^-6 and the force are evaluated in FP32.
+template <int D, bool ENERGY>
+__device__ __forceinline__ void lj32(const PairParam &pp, const Vec<D> &r,
+                                     float (&f)[D], PairAcc &acc) {
+    float rf[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
+    float d2 = rf[0] * rf[0];
+#pragma unroll
+    for (int k = 1; k < D; k++) d2 = fmaf(rf[k], rf[k], d2);
+    // The LJ force is not shifted to zero at the cutoff, so the in/out
+    // decision must not flip with FP32 rounding: within 1e-5 of the cutoff it
+    // is taken in FP64 exactly as the FP64 path takes it.
+    if (d2 > pp.lim_f * 0.99999f) {
+        if (d2 > pp.lim_f * 1.00001f) return;
+        const double d2d = sqnorm<D>(r);
+        if (!(d2d < pp.lim_lo || (!(d2d > pp.lim_hi) && !(d2d / pp.sig2 > pp.cut2))))
+            return;
+    }
+    const float idsq = __builtin_amdgcn_rcpf(d2);
+    const float ir = pp.sig2_f * idsq;
+    const float rsix = ir * ir * ir;
+    const float fmagR = pp.p6_f * rsix * (rsix - 1.0f);
+    const float fs = fmagR * idsq;
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = fmaf(fs, rf[k], f[k]);
+    if (ENERGY) {
+        const float mid = 1.0f - rsix;
+        const double e = (double)(pp.eps_f * (mid * mid) - pp.p5_f);
+        acc.e += e;
+        acc.w += (double)fmagR;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int b = 0; b < D; b++)
+                acc.s[a * D + b] += (double)(rf[a] * (rf[b] * fs));
+        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
+        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
+    }
+}
+
+template <int D, bool ENERGY, bool HARM>
+__device__ __forceinline__ void rep32(const PairParam &pp, const Vec<D> &r,
+                                      float (&f)[D], PairAcc &acc) {
+    float rf[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
+    float d2 = rf[0] * rf[0];
+#pragma unroll
+    for (int k = 1; k < D; k++) d2 = fmaf(rf[k], rf[k], d2);
+    if (d2 > pp.sig2_f * 0.99999f) {  // contact decided in FP64 near sigma
+        if (d2 > pp.sig2_f * 1.00001f) return;
+        if (sqnorm<D>(r) > pp.sig2) return;
+    }
+    const float R = __builtin_sqrtf(d2);
+    const float x = fmaxf(1.0f - R * pp.isig_f, 0.0f);
+    const float px = HARM ? x : powf(x, pp.p6_f);
+    const float fs = pp.eps_f * px * pp.isig_f * __builtin_amdgcn_rcpf(R);
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = fmaf(rf[k], fs, f[k]);
+    if (ENERGY) {
+        const double e =
+            (double)(pp.eps_f * (HARM ? x * x : powf(x, pp.p5_f)) / pp.p5_f);
+        acc.e += e;
+        float rfs = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; k++) rfs = fmaf(rf[k], rf[k] * fs, rfs);
+        acc.w += (double)rfs;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int b = 0; b < D; b++)
+                acc.s[a * D + b] += (double)(rf[a] * (rf[b] * fs));
+        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
+        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
+    }
+}
+
+// ------------------------------------------------------------ step kernels
+__device__ __forceinline__ bool is_fixed(double m) { return m <= 0 || isinf(m); }
+
+// First half of a step for one atom: the reference's first loop.
+template <int D, bool DAMPED>
+__device__ __forceinline__ void drift_atom(Vec<D> &x, double (&v)[D],
+                                           const double (&a)[D], bool fixed,
+                                           const StepCoef &c) {
+    if (fixed) {
+#pragma unroll
+        for (int k = 0; k < D; k++) v[k] = 0.0;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        if (!DAMPED) {
+            // x += v*dt + a*(dt*dt/2) ; v += a*(dt/2)
+            x.c[k] = x.c[k] + (v[k] * c.k0 + a[k] * c.k1);
+            v[k] = v[k] + a[k] * c.k2;
+        } else {
+            // x += v*(c1 dt) + a*(c2 dt dt) ; v = v*c0 + a*(dt (c1 - c2))
+            x.c[k] = x.c[k] + (v[k] * c.k0 + a[k] * c.k1);
+            v[k] = v[k] * c.k3 + a[k] * c.k2;
+        }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void skin_atom(const Vec<D> &x, const void *xref, int i,
+                                          Top2 &t, double &nonfinite) {
+    const Vec<D> x0 =
+        PosStore<D>::load(static_cast<const typename PosStore<D>::T *>(xref)[i]);
+    Vec<D> d;
+#pragma unroll
+    for (int k = 0; k < D; k++) d.c[k] = x.c[k] - x0.c[k];
+    const double dist = sqrt(sqnorm<D>(d));
+    t.add(dist);
+    if (!isfinite(dist)) nonfinite += 1.0;
+}
+
+// One pair visit: dispatch on kind and precision.
+template <int D, int KIND, int PREC, bool ENERGY, bool HARM>
+__device__ __forceinline__ void visit(const PairParam &pp, const Vec<D> &r,
+                                      double (&f64)[D], float (&f32)[D],
+                                      PairAcc &acc) {
+    if (PREC == kFP64) {
+        if (KIND == kPairLJ)
+            lj64<D, ENERGY>(pp, r, f64, acc);
+        else
+            rep64<D, ENERGY, HARM>(pp, r, f64, acc);
+    } else {
+        if (KIND == kPairLJ)
+            lj32<D, ENERGY>(pp, r, f32, acc);
+        else
+            rep32<D, ENERGY, HARM>(pp, r, f32, acc);
+    }
+}
+
+// TPA lanes share one atom: lane `sub` visits list entries sub, sub + TPA, ...
+// The list is stored in groups of TPA entries per atom (entry q of slot i at
+// ((q / TPA) * npad + i) * TPA + q % TPA), so the 64 lanes of a wave read
+// 64 consecutive words per step of the loop.  Two entries per lane are in
+// flight at a time (both list words, then both position gathers) to hide
+// memory latency.  The TPA partial forces are summed with a fixed butterfly,
+// so the result is deterministic.
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, bool ENERGY,
+          int TPA>
+__global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
+                                                  PairView P, Box box, StepCoef c,
+                                                  ReduceBuf R) {
+    using PT = typename PosStore<D>::T;
+    constexpr bool DRIFT =
+        (STAGE == kStageVerletKickDrift || STAGE == kStageDampedKickDrift);
+    constexpr bool DAMPED =
+        (STAGE == kStageDampedKick || STAGE == kStageDampedKickDrift);
+    const int gt = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int i = gt / TPA;
+    const int sub = gt % TPA;
+    const bool act = i < A.n;
+    PairAcc acc;
+    Top2 top;
+    double nonfinite = 0.0;
+    double fi[D];
+    float f32[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        fi[k] = 0.0;
+        f32[k] = 0.0f;
+    }
+
+    const PT *pos = static_cast<const PT *>(A.pos_in);
+    Vec<D> xi;
+    if (act) xi = PosStore<D>::load(pos[i]);
+    const bool member = act && A.rank[i] >= 0;
+    if (member && Lv.built) {
+        const int cnt = Lv.ncount[i];
+        const size_t stride = (size_t)Lv.npad * TPA;
+        const uint32_t *nb = Lv.nbr + (size_t)i * TPA + sub;
+        const int tb = Lv.type_bits;
+        const uint32_t tmask = (1u << tb) - 1u;
+        const PairParam *row = P.tab;
+        PairParam p0;
+        if (SINGLE)
+            p0 = P.tab[0];
+        else
+            row = P.tab + (size_t)A.type[i] * P.ntypes;
+        int q = sub;
+        size_t off = 0;
+        for (; q + TPA < cnt; q += 2 * TPA, off += 2 * stride) {
+            const uint32_t e0 = nb[off];
+            const uint32_t e1 = nb[off + stride];
+            const PT pj0 = pos[e0 >> tb];
+            const PT pj1 = pos[e1 >> tb];
+            const Vec<D> r0 = diff<D>(xi, PosStore<D>::load(pj0), box);
+            visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e0 & tmask], r0, fi,
+                                               f32, acc);
+            const Vec<D> r1 = diff<D>(xi, PosStore<D>::load(pj1), box);
+            visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e1 & tmask], r1, fi,
+                                               f32, acc);
+        }
+        if (q < cnt) {
+            const uint32_t e0 = nb[off];
+            const Vec<D> r0 = diff<D>(xi, PosStore<D>::load(pos[e0 >> tb]), box);
+            visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e0 & tmask], r0, fi,
+                                               f32, acc);
+        }
+    }
+    if (PREC != kFP64) {
+#pragma unroll
+        for (int k = 0; k < D; k++) fi[k] = (double)f32[k];
+    }
+    // sum the TPA partial forces of the atom (all lanes take part)
+#pragma unroll
+    for (int o = 1; o < TPA; o <<= 1) {
+#pragma unroll
+        for (int k = 0; k < D; k++) fi[k] += __shfl_xor(fi[k], o);
+    }
+
+    if (act && sub == 0 && STAGE != kStageSums) {
+#pragma unroll
+        for (int k = 0; k < D; k++) A.f[k][i] = fi[k];
+        if (STAGE != kStageForceOnly) {
+            const double m = A.m[i];
+            const bool fixed = is_fixed(m);
+            double a[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) a[k] = fixed ? 0.0 : fi[k] / m;
+#pragma unroll
+            for (int k = 0; k < D; k++) A.a[k][i] = a[k];
+            if (STAGE != kStageForceA) {
+                double v[D];
+#pragma unroll
+                for (int k = 0; k < D; k++) v[k] = A.v[k][i];
+                if (!fixed) {
+#pragma unroll
+                    for (int k = 0; k < D; k++)
+                        v[k] = v[k] + a[k] * (DAMPED ? c.k4 : c.k2);
+                }
+                if (DRIFT) {
+                    Vec<D> xn = xi;
+                    drift_atom<D, DAMPED>(xn, v, a, fixed, c);
+                    static_cast<PT *>(A.pos_out)[i] = PosStore<D>::make(xn);
+                    if (A.rank[i] >= 0) skin_atom<D>(xn, A.xref, i, top, nonfinite);
+                }
+#pragma unroll
+                for (int k = 0; k < D; k++) A.v[k][i] = v[k];
+            }
+        }
+    }
+
+    if (ENERGY || DRIFT) {
+        double v[kRedWidth];
+#pragma unroll
+        for (int k = 0; k < kRedWidth; k++) v[k] = 0.0;
+        if (ENERGY) {
+            v[kSumU] = 0.5 * acc.e;
+            v[kSumW] = 0.5 * acc.w;
+#pragma unroll
+            for (int k = 0; k < D * D; k++) v[kSumS + k] = 0.5 * acc.s[k];
+            v[kSumContacts] = acc.contacts;
+            v[kSumOverlaps] = acc.overlaps;
+        }
+        v[kNonFinite] = nonfinite;
+        block_partial<ENERGY, DRIFT>(v, top, R);
+    }
+}
+
+// Standalone first half of a step (drift + half kick) and skin test.
+template <int D, bool DAMPED>
+__global__ __launch_bounds__(kBlock) void k_drift(AtomArrays A, StepCoef c,
+                                                  ReduceBuf R) {
+    using PT = typename PosStore<D>::T;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    Top2 top;
+    double nonfinite = 0.0;
+    if (i < A.n) {
+        PT *pos = static_cast<PT *>(A.pos_out);
+        Vec<D> x = PosStore<D>::load(pos[i]);
+        double v[D], a[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            v[k] = A.v[k][i];
+            a[k] = A.a[k][i];
+        }
+        const bool fixed = is_fixed(A.m[i]);
+        drift_atom<D, DAMPED>(x, v, a, fixed, c);
+        if (!fixed) pos[i] = PosStore<D>::make(x);
+#pragma unroll
+        for (int k = 0; k < D; k++) A.v[k][i] = v[k];
+        if (A.rank[i] >= 0) skin_atom<D>(x, A.xref, i, top, nonfinite);
+    }
+    double v[kRedWidth];
+#pragma unroll
+    for (int k = 0; k < kRedWidth; k++) v[k] = 0.0;
+    v[kNonFinite] = nonfinite;
+    block_partial<false, true>(v, top, R);
+}
+
+// Skin test only (after a rebuild, or for update_list(false) on demand).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_skin(AtomArrays A, ReduceBuf R) {
+    using PT = typename PosStore<D>::T;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    Top2 top;
+    double nonfinite = 0.0;
+    if (i < A.n && A.rank[i] >= 0) {
+        const Vec<D> x = PosStore<D>::load(static_cast<const PT *>(A.pos_in)[i]);
+        skin_atom<D>(x, A.xref, i, top, nonfinite);
+    }
+    double v[kRedWidth];
+#pragma unroll
+    for (int k = 0; k < kRedWidth; k++) v[k] = 0.0;
+    v[kNonFinite] = nonfinite;
+    block_partial<false, true>(v, top, R);
+}
+
+// Atom-wise observables (AtomGroup::kinetic_energy / momentum / mass,
+// src/box.cpp:239-260, 401-411) and force statistics.
+template <int D, int MODE>
+__global__ __launch_bounds__(kBlock) void k_atom_sums(AtomArrays A, Vec<D> ov,
+                                                      ReduceBuf R) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v[kRedWidth];
+#pragma unroll
+    for (int k = 0; k < kRedWidth; k++) v[k] = 0.0;
+    Top2 top;
+    if (i < A.n) {
+        const double m = A.m[i];
+        if (MODE == 0) {
+            if (!(m == 0 || isinf(m))) {
+                double c[D];
+#pragma unroll
+                for (int k = 0; k < D; k++) c[k] = A.v[k][i] - ov.c[k];
+                double dd;
+                if constexpr (D == 3)
+                    dd = (c[0] * c[0] + c[1] * c[1]) + c[2] * c[2];
+                else
+                    dd = c[0] * c[0] + c[1] * c[1];
+                v[0] = m / 2 * dd;
+            }
+            if (!is_fixed(m)) {
+#pragma unroll
+                for (int k = 0; k < D; k++) v[1 + k] = A.v[k][i] * m;
+                v[4] = m;
+            }
+        } else {
+            Vec<D> f;
+#pragma unroll
+            for (int k = 0; k < D; k++) f.c[k] = A.f[k][i];
+            const double fn = sqrt(sqnorm<D>(f));
+            v[0] = fn;
+            top.add(fn);
+        }
+    }
+    block_partial<true, MODE == 1>(v, top, R);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_upload(AtomArrays A, int what,
+                                                   const int32_t *__restrict__ orig,
+                                                   const double *__restrict__ src) {
+    using PT = typename PosStore<D>::T;
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= A.n) return;
+    const size_t o = (size_t)orig[s] * D;
+    if (what == 0) {
+        Vec<D> x;
+#pragma unroll
+        for (int k = 0; k < D; k++) x.c[k] = src[o + k];
+        static_cast<PT *>(A.pos_out)[s] = PosStore<D>::make(x);
+        return;
+    }
+    double *const *dst = what == 1 ? A.v : (what == 2 ? A.a : A.f);
+#pragma unroll
+    for (int k = 0; k < D; k++) dst[k][s] = src[o + k];
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_download(AtomArrays A, int what,
+                                                     const int32_t *__restrict__ orig,
+                                                     double *__restrict__ dst) {
+    using PT = typename PosStore<D>::T;
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= A.n) return;
+    const size_t o = (size_t)orig[s] * D;
+    if (what == 0) {
+        const Vec<D> x = PosStore<D>::load(static_cast<const PT *>(A.pos_in)[s]);
+#pragma unroll
+        for (int k = 0; k < D; k++) dst[o + k] = x.c[k];
+        return;
+    }
+    double *const *src = what == 1 ? A.v : (what == 2 ? A.a : A.f);
+#pragma unroll
+    for (int k = 0; k < D; k++) dst[o + k] = src[k][s];
+}
+
+// ======================================================== host launchers
+int reduce_blocks(int n) { return n <= 0 ? 1 : (n + kBlock - 1) / kBlock; }
+int force_blocks(int n, int tpa) { return reduce_blocks(n * tpa); }
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, int TPA>
+static void launch_force_tt(bool energy, const AtomArrays &A, const ListView &Lv,
+                            const PairView &P, const Box &box, const StepCoef &c,
+                            const ReduceBuf &R, hipStream_t s) {
+    constexpr bool DRIFT =
+        (STAGE == kStageVerletKickDrift || STAGE == kStageDampedKickDrift);
+    const int nb = force_blocks(A.n, TPA);
+    dim3 g(nb), b(kBlock);
+    if (energy) {
+        hipLaunchKernelGGL((k_force<D, KIND, PREC, SINGLE, HARM, STAGE, true, TPA>), g,
+                           b, 0, s, A, Lv, P, box, c, R);
+        launch_finalize<true, DRIFT>(nb, R, s);
+    } else {
+        hipLaunchKernelGGL((k_force<D, KIND, PREC, SINGLE, HARM, STAGE, false, TPA>), g,
+                           b, 0, s, A, Lv, P, box, c, R);
+        if (DRIFT) launch_finalize<false, true>(nb, R, s);
+    }
+}
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE>
+static void launch_force_t(bool energy, const AtomArrays &A, const ListView &Lv,
+                           const PairView &P, const Box &box, const StepCoef &c,
+                           const ReduceBuf &R, hipStream_t s) {
+    switch (Lv.tpa) {
+    case 1:
+        launch_force_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 1>(energy, A, Lv, P, box, c,
+                                                               R, s);
+        break;
+    case 2:
+        launch_force_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 2>(energy, A, Lv, P, box, c,
+                                                               R, s);
+        break;
+    case 8:
+        launch_force_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 8>(energy, A, Lv, P, box, c,
+                                                               R, s);
+        break;
+    default:
+        launch_force_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 4>(energy, A, Lv, P, box, c,
+                                                               R, s);
+        break;
+    }
+}
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM>
+static void launch_force_s(int stage, bool energy, const AtomArrays &A,
+                           const ListView &Lv, const PairView &P, const Box &box,
+                           const StepCoef &c, const ReduceBuf &R, hipStream_t s) {
+    switch (stage) {
+#define PARM_STAGE(S)                                                          \
+    case S:                                                                    \
+        launch_force_t<D, KIND, PREC, SINGLE, HARM, S>(energy, A, Lv, P, box,  \
+                                                       c, R, s);               \
+        break;
+        PARM_STAGE(kStageForceOnly)
+        PARM_STAGE(kStageForceA)
+        PARM_STAGE(kStageVerletKick)
+        PARM_STAGE(kStageVerletKickDrift)
+        PARM_STAGE(kStageDampedKick)
+        PARM_STAGE(kStageDampedKickDrift)
+        PARM_STAGE(kStageSums)
+#undef PARM_STAGE
+    }
+}
+
+template <int D, int KIND, int PREC>
+static void launch_force_k(int stage, bool energy, const AtomArrays &A,
+                           const ListView &Lv, const PairView &P, const Box &box,
+                           const StepCoef &c, const ReduceBuf &R, hipStream_t s) {
+    if (KIND == kPairRepulsion && P.harmonic) {
+        if (P.single)
+            launch_force_s<D, KIND, PREC, true, true>(stage, energy, A, Lv, P, box, c,
+                                                      R, s);
+        else
+            launch_force_s<D, KIND, PREC, false, true>(stage, energy, A, Lv, P, box,
+                                                       c, R, s);
+    } else {
+        if (P.single)
+            launch_force_s<D, KIND, PREC, true, false>(stage, energy, A, Lv, P, box,
+                                                       c, R, s);
+        else
+            launch_force_s<D, KIND, PREC, false, false>(stage, energy, A, Lv, P, box,
+                                                        c, R, s);
+    }
+}
+
+template <int D>
+static void launch_force_d(int precision, int stage, bool energy, const AtomArrays &A,
+                           const ListView &Lv, const PairView &P, const Box &box,
+                           const StepCoef &c, const ReduceBuf &R, hipStream_t s) {
+    if (P.kind == kPairRepulsion) {
+        if (precision == kFP64)
+            launch_force_k<D, kPairRepulsion, kFP64>(stage, energy, A, Lv, P, box, c,
+                                                     R, s);
+        else
+            launch_force_k<D, kPairRepulsion, kMixed>(stage, energy, A, Lv, P, box,
+                                                      c, R, s);
+    } else {
+        if (precision == kFP64)
+            launch_force_k<D, kPairLJ, kFP64>(stage, energy, A, Lv, P, box, c, R, s);
+        else
+            launch_force_k<D, kPairLJ, kMixed>(stage, energy, A, Lv, P, box, c, R,
+                                               s);
+    }
+}
+
+hipError_t launch_force(int ndim, int precision, int stage, bool energy,
+                        const AtomArrays &A, const ListView &Lv, const PairView &P,
+                        const Box &box, const StepCoef &c, const ReduceBuf &R,
+                        hipStream_t s) {
+    if (A.n <= 0) return hipSuccess;
+    if (ndim == 3)
+        launch_force_d<3>(precision, stage, energy, A, Lv, P, box, c, R, s);
+    else
+        launch_force_d<2>(precision, stage, energy, A, Lv, P, box, c, R, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_drift(int ndim, int stage, const AtomArrays &A, const StepCoef &c,
+                        const ReduceBuf &R, hipStream_t s) {
+    if (A.n <= 0) return hipSuccess;
+    dim3 g(reduce_blocks(A.n)), b(kBlock);
+    const bool damped = (stage == kStageDampedKick || stage == kStageDampedKickDrift);
+    if (ndim == 3) {
+        if (damped)
+            hipLaunchKernelGGL((k_drift<3, true>), g, b, 0, s, A, c, R);
+        else
+            hipLaunchKernelGGL((k_drift<3, false>), g, b, 0, s, A, c, R);
+    } else {
+        if (damped)
+            hipLaunchKernelGGL((k_drift<2, true>), g, b, 0, s, A, c, R);
+        else
+            hipLaunchKernelGGL((k_drift<2, false>), g, b, 0, s, A, c, R);
+    }
+    launch_finalize<false, true>((int)g.x, R, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_atom_sums(int ndim, int mode, const AtomArrays &A,
+                            const double *originv, const ReduceBuf &R, hipStream_t s) {
+    dim3 g(reduce_blocks(A.n)), b(kBlock);
+    if (ndim == 3) {
+        Vec<3> ov{{0, 0, 0}};
+        if (originv)
+            for (int k = 0; k < 3; k++) ov.c[k] = originv[k];
+        if (mode == 0)
+            hipLaunchKernelGGL((k_atom_sums<3, 0>), g, b, 0, s, A, ov, R);
+        else
+            hipLaunchKernelGGL((k_atom_sums<3, 1>), g, b, 0, s, A, ov, R);
+    } else {
+        Vec<2> ov{{0, 0}};
+        if (originv)
+            for (int k = 0; k < 2; k++) ov.c[k] = originv[k];
+        if (mode == 0)
+            hipLaunchKernelGGL((k_atom_sums<2, 0>), g, b, 0, s, A, ov, R);
+        else
+            hipLaunchKernelGGL((k_atom_sums<2, 1>), g, b, 0, s, A, ov, R);
+    }
+    if (mode == 0)
+        launch_finalize<true, false>((int)g.x, R, s);
+    else
+        launch_finalize<true, true>((int)g.x, R, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_upload(int ndim, int what, const AtomArrays &A, const int32_t *orig,
+                         const double *src, hipStream_t s) {
+    if (A.n <= 0) return hipSuccess;
+    dim3 g(reduce_blocks(A.n)), b(kBlock);
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_upload<3>, g, b, 0, s, A, what, orig, src);
+    else
+        hipLaunchKernelGGL(k_upload<2>, g, b, 0, s, A, what, orig, src);
+    return hipGetLastError();
+}
+
+hipError_t launch_download(int ndim, int what, const AtomArrays &A,
+                           const int32_t *orig, double *dst, hipStream_t s) {
+    if (A.n <= 0) return hipSuccess;
+    dim3 g(reduce_blocks(A.n)), b(kBlock);
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_download<3>, g, b, 0, s, A, what, orig, dst);
+    else
+        hipLaunchKernelGGL(k_download<2>, g, b, 0, s, A, what, orig, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_skin(int ndim, const AtomArrays &A, const ReduceBuf &R,
+                       hipStream_t s) {
+    if (A.n <= 0) return hipSuccess;
+    dim3 g(reduce_blocks(A.n)), b(kBlock);
+    if (ndim == 3)
+        hipLaunchKernelGGL((k_skin<3>), g, b, 0, s, A, R);
+    else
+        hipLaunchKernelGGL((k_skin<2>), g, b, 0, s, A, R);
+    launch_finalize<false, true>((int)g.x, R, s);
+    return hipGetLastError();
+}
+
+}  // namespace parm

@@ -1,0 +1,292 @@
+"""Device path vs the CPU oracle, through the C ABI (libparm_hip.so).
+
+Bars (north-star and SURVEY parity rules):
+  * neighbour-pair set: bit-identical (same pairs, same reference order);
+  * FP64 forces: |F_dev - F_ref| <= 1e-12 * sum_j |f_ij| per atom;
+  * FP64 energy / virial / stress: 1e-12 relative; contacts / overlaps exact;
+  * integrators: the device step reproduces the oracle's trajectory to
+    1e-12 relative over 10 steps (only force summation order differs);
+  * mixed precision: forces within 2e-5 of FP64 (normalised as above).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from parm_amd import capi
+from tests import systems as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _device_or_skip():
+    if capi.device_count() < 1:
+        pytest.fail("no GPU visible: the -m gpu tests need an MI355X")
+
+
+def _table(kind, p):
+    return O.PairTable(kind, p["eps"], p["sigma"], p["third"], p["types"])
+
+
+def _make(kind, x, L, p, skin, prec=capi.FP64, masses=None, members=None):
+    n, ndim = x.shape
+    s = capi.System(ndim, np.ones(n) if masses is None else masses, prec)
+    s.set_box(L)
+    if members is None:
+        s.nlist_set(skin, p["diam"])
+    else:
+        s.nlist_set(skin, p["diam"][members], members)
+    if kind == O.LJ_ATTRACT_REPULSE:
+        s.set_lj(p["eps"], p["sigma"], p["third"], p["types"])
+    else:
+        s.set_repulsion(p["eps"], p["sigma"], p["third"], p["types"])
+    s.set_state(x=x)
+    return s
+
+
+CASES = [
+    # kind, ndim, n, rho, skin, bidisperse
+    (O.LJ_ATTRACT_REPULSE, 3, 1000, 0.8, 0.4, False),
+    (O.LJ_ATTRACT_REPULSE, 3, 4096, 0.8, 0.3, False),
+    (O.REPULSION, 3, 3000, 1.0, 0.2, True),
+    (O.REPULSION, 2, 5000, 1.0, 0.2, True),
+    (O.LJ_ATTRACT_REPULSE, 2, 2000, 0.6, 0.3, False),
+]
+
+
+@pytest.mark.parametrize("kind,ndim,n,rho,skin,bi", CASES)
+def test_pair_set_bit_identical(kind, ndim, n, rho, skin, bi):
+    _device_or_skip()
+    print(f"pair-set case kind={kind} ndim={ndim} n={n}", flush=True)
+    x, L = S.jittered_lattice(n, rho, ndim, seed=n + ndim)
+    p = S.bidisperse_repulsion(n, n) if bi else S.lj_params(n)
+    s = _make(kind, x, L, p, skin)
+    assert s.update_list(True)
+    f_dev, l_dev = s.pairs()
+    f_ref, l_ref = O.nlist(x, L, p["diam"], skin, brute=(n <= 3000))
+    assert s.numpairs() == len(f_ref)
+    assert np.array_equal(f_dev, f_ref) and np.array_equal(l_dev, l_ref)
+    # the device keeps the caller's positions bit for bit through its sort
+    assert np.array_equal(s.positions(), x)
+
+
+def test_pair_set_subset_members_and_tiny_box():
+    _device_or_skip()
+    rng = np.random.default_rng(4)
+    # box smaller than 3 list radii (cells collapse), components near L/2
+    n = 300
+    L = np.array([4.0, 3.7, 4.3])
+    x = rng.uniform(-10, 10, (n, 3))
+    x[1] = x[0] + np.array([2.0, 0.1, 0.0])      # |dx| = L/2 exactly (tie)
+    x[2] = x[0] + np.array([6.0, 0.0, 0.2])      # L + L/2
+    members = rng.permutation(n)[:250].astype(np.int32)
+    members = np.concatenate([[0, 1, 2], members[~np.isin(members, [0, 1, 2])]]).astype(np.int32)
+    p = S.bidisperse_repulsion(n, 3)
+    s = _make(O.REPULSION, x, L, p, 0.45, members=members)
+    s.update_list(True)
+    f_dev, l_dev = s.pairs()
+    f_ref, l_ref = O.nlist(x, L, p["diam"][members], 0.45, member=members, brute=True)
+    assert np.array_equal(f_dev, f_ref) and np.array_equal(l_dev, l_ref)
+
+
+@pytest.mark.parametrize("kind,ndim,n,rho,skin,bi", CASES)
+def test_fp64_forces_energy_virial(kind, ndim, n, rho, skin, bi):
+    _device_or_skip()
+    print(f"forces case kind={kind} ndim={ndim} n={n}", flush=True)
+    x, L = S.jittered_lattice(n, rho, ndim, seed=7 * n + ndim, jitter=0.12)
+    p = S.bidisperse_repulsion(n, n) if bi else S.lj_params(n)
+    s = _make(kind, x, L, p, skin)
+    s.update_list(True)
+    s.set_forces(False)
+    f_dev = s.forces()
+    first, last = O.nlist(x, L, p["diam"], skin)
+    ref = O.forces(x, L, _table(kind, p), first, last)
+    err = S.normalised_force_error(f_dev, ref["f"], ref["fabs"])
+    assert err <= 1e-12, err
+    sums = s.pair_sums()
+    assert abs(sums["U"] - ref["U"]) <= 1e-12 * max(abs(ref["U"]), 1e-300) + 1e-13
+    assert abs(sums["W"] - ref["W"]) <= 1e-12 * max(abs(ref["W"]), 1e-300) + 1e-13
+    scale = np.abs(ref["S"]).max() + 1e-300
+    assert np.abs(sums["S"] - ref["S"]).max() <= 1e-12 * scale
+    assert sums["contacts"] == ref["contacts"]
+    assert sums["overlaps"] == ref["overlaps"]
+
+
+def test_repulsion_general_exponent_and_multi_type_lj():
+    _device_or_skip()
+    n = 2000
+    x, L = S.jittered_lattice(n, 0.9, 3, seed=5, jitter=0.15)
+    p = S.bidisperse_repulsion(n, 5, eps=(1.0, 2.0), exponent=(2.5, 3.0))
+    s = _make(O.REPULSION, x, L, p, 0.3)
+    s.update_list(True)
+    s.set_forces(True)
+    first, last = O.nlist(x, L, p["diam"], 0.3)
+    ref = O.forces(x, L, _table(O.REPULSION, p), first, last)
+    assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
+    # two-type LJ with an eps <= 0 (purely repulsive) type pair
+    types = (np.arange(n) % 2).astype(np.int32)
+    q = dict(eps=[[1.0, 0.7], [0.7, -1.5]], sigma=[1.0, 1.2], third=[2.5, 2.2],
+             types=types, diam=np.where(types == 1, 1.2 * 2.2, 2.5))
+    s2 = _make(O.LJ_ATTRACT_REPULSE, x, L, q, 0.3)
+    s2.update_list(True)
+    s2.set_forces(False)
+    first, last = O.nlist(x, L, q["diam"], 0.3)
+    ref = O.forces(x, L, _table(O.LJ_ATTRACT_REPULSE, q), first, last)
+    assert S.normalised_force_error(s2.forces(), ref["f"], ref["fabs"]) <= 1e-12
+    assert abs(s2.pair_sums()["U"] - ref["U"]) <= 1e-12 * abs(ref["U"])
+
+
+def _oracle_system(kind, x, L, p, skin, masses):
+    n, ndim = x.shape
+    return O.System(masses, L, _table(kind, p), p["diam"], skin, ndim=ndim)
+
+
+@pytest.mark.parametrize("kind,ndim,n,integrator", [
+    (O.LJ_ATTRACT_REPULSE, 3, 1000, "verlet"),
+    (O.REPULSION, 2, 4000, "damped"),
+    (O.REPULSION, 3, 2000, "verlet"),
+])
+def test_trajectory_matches_oracle(kind, ndim, n, integrator):
+    _device_or_skip()
+    print(f"trajectory case kind={kind} ndim={ndim} n={n} {integrator}", flush=True)
+    rho = 0.8 if kind == O.LJ_ATTRACT_REPULSE else 1.0
+    x, L = S.jittered_lattice(n, rho, ndim, seed=11 + n)
+    p = S.lj_params(n) if kind == O.LJ_ATTRACT_REPULSE else S.bidisperse_repulsion(n, n)
+    masses = np.ones(n)
+    masses[3] = np.inf  # a fixed atom
+    v = S.gaussian_velocities(n, ndim, 1.0, seed=n)
+    skin = 0.4 if kind == O.LJ_ATTRACT_REPULSE else 0.2
+    dev = _make(kind, x, L, p, skin, masses=masses)
+    dev.set_state(v=v)
+    ref = _oracle_system(kind, x, L, p, skin, masses)
+    ref.set_state(x=x, v=v, a=np.zeros_like(x))
+    dev.initialize()
+    ref.initialize()
+    dt = 0.005 if integrator == "verlet" else 0.01
+    # chunks of 1, 2, 3, 4 steps: single-step calls run drift + force kernels,
+    # longer calls the fused force+kick+drift kernel between them
+    for step, k in enumerate((1, 2, 3, 4)):
+        if integrator == "verlet":
+            dev.verlet(dt, k)
+            ref.verlet(dt, k)
+        else:
+            dev.damped(dt, 0.5, k)
+            ref.damped(dt, 0.5, k)
+        xd, vd, ad, fd = dev.state()
+        xr, vr, ar, fr = ref.state()
+        assert dev.which() == ref.which(), step
+        assert np.abs(xd - xr).max() <= 1e-12 * max(1.0, np.abs(xr).max()), step
+        assert np.abs(vd - vr).max() <= 1e-11 * max(1.0, np.abs(vr).max()), step
+        assert np.abs(fd - fr).max() <= 1e-10 * max(1.0, np.abs(fr).max()), step
+    fd_, ld_ = dev.pairs()
+    fr_, lr_ = ref.pairs()
+    assert np.array_equal(fd_, fr_) and np.array_equal(ld_, lr_)
+    assert abs(dev.kinetic_energy() - ref.kinetic_energy()) <= 1e-12 * ref.kinetic_energy()
+
+
+def test_skin_rebuild_schedule_matches_oracle():
+    _device_or_skip()
+    n = 1500
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=21)
+    p = S.lj_params(n)
+    v = S.gaussian_velocities(n, 3, 2.0, seed=3)
+    dev = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.15)
+    dev.set_state(v=v)
+    ref = _oracle_system(O.LJ_ATTRACT_REPULSE, x, L, p, 0.15, np.ones(n))
+    ref.set_state(x=x, v=v, a=np.zeros_like(x))
+    dev.initialize()
+    ref.initialize()
+    whiches = []
+    for k in (1, 1, 5, 13, 40):  # fused multi-step calls rebuild mid-call
+        dev.verlet(0.004, k)
+        ref.verlet(0.004, k)
+        whiches.append((dev.which(), ref.which()))
+    assert all(a == b for a, b in whiches), whiches
+    assert whiches[-1][0] > 3  # several rebuilds happened
+    xd, vd, _, _ = dev.state()
+    xr, vr, _, _ = ref.state()
+    assert np.abs(xd - xr).max() <= 1e-10 * np.abs(xr).max()
+    fd_, ld_ = dev.pairs()
+    fr_, lr_ = ref.pairs()
+    assert np.array_equal(fd_, fr_) and np.array_equal(ld_, lr_)
+
+
+def test_mixed_precision_force_error():
+    _device_or_skip()
+    n = 4096
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=99, jitter=0.1)
+    p = S.lj_params(n)
+    s64 = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3)
+    s32 = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3, prec=capi.MIXED)
+    for s in (s64, s32):
+        s.update_list(True)
+        s.set_forces(False)
+    first, last = O.nlist(x, L, p["diam"], 0.3)
+    ref = O.forces(x, L, _table(O.LJ_ATTRACT_REPULSE, p), first, last)
+    err = S.normalised_force_error(s32.forces(), ref["f"], ref["fabs"])
+    assert err <= 2e-5, err
+    u32 = s32.pair_sums()["U"]
+    assert abs(u32 - ref["U"]) <= 1e-5 * abs(ref["U"])
+
+
+def test_errors_are_reported():
+    _device_or_skip()
+    s = capi.System(3, np.ones(4))
+    with pytest.raises(ValueError):
+        s.set_box([1.0, -1.0, 1.0])
+    with pytest.raises(ValueError):
+        s.nlist_set(0.3, [1.0, 1.0], members=[1, 1])
+    with pytest.raises(RuntimeError):
+        s.set_forces()  # no box / list / interaction yet
+    with pytest.raises(ValueError):
+        s.damped(0.0, 1.0, 1)
+
+
+def test_workload_push_apart_min_separation():
+    _device_or_skip()
+    from parm_amd import workloads
+    x, v, L = workloads.lj_liquid(3000, rho=0.8, T=1.0, min_sep=0.8, seed=5)
+    first, last = O.nlist(x, L, np.full(3000, 0.8), 0.0, brute=True)
+    assert len(first) == 0  # no pair closer than 0.8
+    assert np.abs(v.sum(0)).max() < 1e-9
+    assert abs(np.sum(v * v) / (3 * 3000 - 3) - 1.0) < 1e-12
+
+
+def test_fp64_energy_conservation_large():
+    """NVE at N = 65,536, 500 steps, dt = 0.002: a size-independent property."""
+    _device_or_skip()
+    from parm_amd import workloads
+    n = 65536
+    x, v, L = workloads.lj_liquid(n, seed=9)
+    s = _make(O.LJ_ATTRACT_REPULSE, x, L, S.lj_params(n), 0.3)
+    s.set_state(v=v, a=np.zeros_like(x))
+    s.initialize()
+    s.verlet(0.002, 200)  # let the initial push-apart configuration relax
+    e0 = s.pair_sums()["U"] + s.kinetic_energy()
+    es = []
+    for _ in range(5):
+        s.verlet(0.002, 100)
+        es.append(s.pair_sums()["U"] + s.kinetic_energy())
+    drift = max(abs(e - e0) for e in es) / abs(e0)
+    assert drift < 1e-3, drift
+    p, m = s.momentum()
+    assert np.abs(p).max() < 1e-9 * n
+
+
+@pytest.mark.parametrize("tpa", ["1", "2", "4", "8"])
+def test_every_lanes_per_atom_layout(tpa, monkeypatch):
+    """The force kernel's lanes-per-atom choices (autotuned at run time) all
+    reproduce the oracle's forces and pair set."""
+    _device_or_skip()
+    monkeypatch.setenv("PARM_TPA", tpa)
+    n = 3000
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=int(tpa))
+    p = S.lj_params(n)
+    s = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3)
+    s.update_list(True)
+    s.set_forces(False)
+    first, last = O.nlist(x, L, p["diam"], 0.3)
+    ref = O.forces(x, L, _table(O.LJ_ATTRACT_REPULSE, p), first, last)
+    assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
+    f_dev, l_dev = s.pairs()
+    assert np.array_equal(f_dev, first) and np.array_equal(l_dev, last)
+    assert abs(s.pair_sums()["U"] - ref["U"]) <= 1e-12 * abs(ref["U"])
