@@ -142,6 +142,10 @@ int parm_pair_repulsion(parm_system *sys, int ntypes, const double *eps,
  * if and_a, a = f/m (a = 0 for m <= 0 or m = inf). */
 int parm_set_forces(parm_system *sys, int and_a);
 
+/* NListed::set_forces (src/interaction.hpp:2165-2175): f += pair forces over
+ * the current list (a is not touched). */
+int parm_add_pair_forces(parm_system *sys);
+
 /* One fused pass over the current list at the current positions:
  *   *U        NListed::energy            (interaction.hpp:2153-2163)
  *   *W        NListed::pressure (sum r.f) (interaction.hpp:2247-2261)

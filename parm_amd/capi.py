@@ -28,7 +28,7 @@ EXPORTS = (
     "parm_nlist_set", "parm_nlist_update", "parm_nlist_which",
     "parm_nlist_numpairs", "parm_nlist_pairs",
     "parm_pair_lj_attract_repulse", "parm_pair_repulsion",
-    "parm_set_forces", "parm_pair_sums", "parm_kinetic_energy", "parm_momentum",
+    "parm_set_forces", "parm_add_pair_forces", "parm_pair_sums", "parm_kinetic_energy", "parm_momentum",
     "parm_force_stats", "parm_initialize", "parm_verlet_steps",
     "parm_damped_steps", "parm_synchronize", "parm_profile_enable",
     "parm_profile_read",
@@ -83,6 +83,7 @@ def lib():
             "parm_pair_lj_attract_repulse": (i32, [_vp, i32, _dp, _dp, _dp, _ip]),
             "parm_pair_repulsion": (i32, [_vp, i32, _dp, _dp, _dp, _ip]),
             "parm_set_forces": (i32, [_vp, i32]),
+            "parm_add_pair_forces": (i32, [_vp]),
             "parm_pair_sums": (i32, [_vp, _dp, _dp, _dp, C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint64)]),
             "parm_kinetic_energy": (i32, [_vp, _dp, _dp]),
@@ -227,6 +228,9 @@ class System:
 
     def set_forces(self, and_a=True):
         _check(lib().parm_set_forces(self.h, int(bool(and_a))))
+
+    def add_pair_forces(self):
+        _check(lib().parm_add_pair_forces(self.h))
 
     def pair_sums(self):
         U, W = C.c_double(0), C.c_double(0)

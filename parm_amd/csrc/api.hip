@@ -610,6 +610,8 @@ __global__ void k_fill_i32(int n, int32_t *p, int32_t v) {
     if (i < n) p[i] = v;
 }
 
+int relayout(parm_system *s);
+
 int build_pair_table(parm_system *s, int kind, int ntypes, const double *eps,
                      const double *sigma, const double *third, const int32_t *atype) {
     if (ntypes < 1 || ntypes > 64)
@@ -687,7 +689,9 @@ int build_pair_table(parm_system *s, int kind, int ntypes, const double *eps,
     s->single = ntypes == 1 ? 1 : 0;
     s->harmonic = (kind == kPairRepulsion && harm) ? 1 : 0;
     s->pair_kind = kind;
-    s->ignorechanged = true;  // list entries carry the type bits
+    // List entries carry the neighbour's type: re-encode the current list
+    // (same pairs, as NListed::update_pairs re-pairs the current list).
+    PARM_CHECK(relayout(s));
     return PARM_OK;
 }
 
@@ -1001,6 +1005,17 @@ int parm_set_forces(parm_system *s, int and_a) {
     PARM_CHECK(require_ready(s));
     PARM_CHECK(set_device(s));
     return set_forces(s, and_a != 0);
+}
+
+int parm_add_pair_forces(parm_system *s) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(require_ready(s));
+    PARM_CHECK(set_device(s));
+    StepCoef c{};
+    PARM_HIP(launch_force(s->ndim, s->prec, kStageForceAdd, false, s->arrays(),
+                          s->listview(), s->pairview(), s->box, c, s->redbuf(kSlotScratch),
+                          s->stream));
+    return PARM_OK;
 }
 
 int parm_pair_sums(parm_system *s, double *U, double *W, double *S, uint64_t *contacts,

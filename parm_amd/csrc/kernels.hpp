@@ -99,6 +99,7 @@ enum Stage : int {
     kStageDampedKick = 4,  // Damped: a = f/m ; v += a dt c2
     kStageDampedKickDrift = 5,
     kStageSums = 6,        // energy / virial / stress only, nothing written
+    kStageForceAdd = 7,    // NListed::set_forces: f += pair forces
 };
 
 struct StepCoef {

@@ -459,8 +459,13 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
 
     if (act && sub == 0 && STAGE != kStageSums) {
 #pragma unroll
-        for (int k = 0; k < D; k++) A.f[k][i] = fi[k];
-        if (STAGE != kStageForceOnly) {
+        for (int k = 0; k < D; k++) {
+            if (STAGE == kStageForceAdd)
+                A.f[k][i] += fi[k];
+            else
+                A.f[k][i] = fi[k];
+        }
+        if (STAGE != kStageForceOnly && STAGE != kStageForceAdd) {
             const double m = A.m[i];
             const bool fixed = is_fixed(m);
             double a[D];
@@ -699,6 +704,7 @@ static void launch_force_s(int stage, bool energy, const AtomArrays &A,
         PARM_STAGE(kStageDampedKick)
         PARM_STAGE(kStageDampedKickDrift)
         PARM_STAGE(kStageSums)
+        PARM_STAGE(kStageForceAdd)
 #undef PARM_STAGE
     }
 }

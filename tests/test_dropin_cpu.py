@@ -1,0 +1,96 @@
+"""Host-side behaviour of the drop-in API (parm_amd.d2 / d3): names,
+signatures, Vec <-> numpy conversion, OriginBox geometry and the reference's
+error conventions.  Without a GPU every device call must raise."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from parm_amd import capi, d2, d3
+
+
+def test_reference_names_exist():
+    for mod in (d2, d3):
+        for name in ("OriginBox", "AtomVec", "AtomID", "Atom", "IDPair", "NeighborList",
+                     "EpsSigExpAtom", "Repulsion", "IEpsSigCutAtom", "LJAttractRepulse",
+                     "Interaction", "StateTracker", "Collection", "CollectionVerlet",
+                     "CollectionDamped", "seed", "rand_vec"):
+            assert hasattr(mod, name), name
+    assert d3.NDIM == 3 and d2.NDIM == 2
+
+
+def test_originbox_diff_is_remainder():
+    rng = np.random.default_rng(0)
+    L = np.array([3.0, 4.5, 7.25])
+    box = d3.OriginBox(L)
+    assert box.V() == np.prod(L)
+    assert abs(box.L() - L.mean()) < 1e-15
+    for _ in range(200):
+        a, b = rng.uniform(-20, 20, 3), rng.uniform(-20, 20, 3)
+        r = box.diff(a, b)
+        assert all(r[k] == O.vec_mod(a[k] - b[k], L[k]) for k in range(3))
+    b2 = d2.OriginBox(5.0)
+    assert np.array_equal(b2.diff([3.0, -2.6], [0.0, 0.0]), [-2.0, 2.4])
+    d3.seed(3)
+    for _ in range(50):
+        p = box.rand_loc()
+        assert np.all(np.abs(p) <= L / 2)
+
+
+def test_atomvec_state_and_iteration():
+    av = d3.AtomVec([1.0, 2.0, 3.0])
+    assert len(av) == 3 and av.size() == 3
+    for i, a in enumerate(av):
+        a.x = [i, 2 * i, 3 * i]
+        a.v = np.array([1.0, 0.0, -1.0]) * i
+    assert np.array_equal(av[2].x, [2, 4, 6]) and av[1].m == 2.0
+    assert np.array_equal(av.get_positions()[:, 1], [0, 2, 4])
+    av.set_velocities(np.ones((3, 3)))
+    assert np.array_equal(av[0].v, [1, 1, 1])
+    with pytest.raises(IndexError):
+        av[3]
+    av2 = d3.AtomVec(5, 1.5)
+    assert av2[4].m == 1.5 and np.array_equal(av2[4].x, [0, 0, 0])
+
+
+def test_duplicate_add_raises_value_error():
+    box = d3.OriginBox(10.0)
+    av = d3.AtomVec([1.0] * 3)
+    nl = d3.NeighborList(box, av, 0.3)
+    nl.add(av[0], 1.0)
+    with pytest.raises(ValueError):
+        nl.add(av[0], 1.0)
+    assert nl.size() == 1
+
+
+def test_one_list_per_atomvec_on_device_path():
+    box = d3.OriginBox(10.0)
+    av = d3.AtomVec([1.0] * 3)
+    first = d3.NeighborList(box, av, 0.3)
+    with pytest.raises(RuntimeError):
+        d3.NeighborList(box, av, 0.3)
+    del first
+    d3.NeighborList(box, av, 0.3)  # released with its owner
+
+
+def test_exclusions_fail_loudly():
+    box = d3.OriginBox(10.0)
+    av = d3.AtomVec([1.0] * 3)
+    nl = d3.NeighborList(box, av, 0.3)
+    with pytest.raises(RuntimeError, match="ignore"):
+        nl.ignore(av[0], av[1])
+
+
+def test_device_calls_raise_without_gpu():
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is visible here")
+    box = d3.OriginBox(10.0)
+    av = d3.AtomVec([1.0] * 4)
+    r = d3.Repulsion(box, av, 0.4)
+    for a in av:
+        r.add(d3.EpsSigExpAtom(a, 1.0, 1.0, 2.0))
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        r.energy(box)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        d3.CollectionVerlet(box, av, 0.01, [r], [r.neighbor_list()])
+    with pytest.raises(ValueError):
+        d3.CollectionDamped(box, av, -1.0, 1.0, [r], [r.neighbor_list()])

@@ -1,0 +1,218 @@
+"""The drop-in API (parm_amd.d3 / d2 and the C++ headers) on the GPU.
+
+RandomHertzianVerletTest below keeps the setup and the assertions of the
+reference's only hot-path test (pyparm/tests.py:218-317): N = 12 bidisperse
+harmonic spheres (Repulsion, eps 1.2, exponent 2), masses sigma^3, phi = 0.3,
+NeighborList skin 0.4, CollectionVerlet dt = 0.01; per-step relative energy
+change < 1e-2, mean temperature within 10 % of 1, std(E)/mean(E) < 1e-2.
+The other tests compare the drop-in path with the CPU oracle.
+"""
+import os
+import subprocess
+import unittest
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from parm_amd import capi, d2, d3
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+class NPTestCase(unittest.TestCase):
+    def assertClose(self, a, b, rtol=1e-5, atol=1e-8):
+        self.assertTrue(np.allclose(a, b, rtol=rtol, atol=atol), f"{a} !~ {b}")
+
+
+class PairedCollectionTest(NPTestCase):
+    seed = None
+    collection_type = None
+    collection_args = None
+    pair_type = None
+    atom_type = None
+    atom_args = None
+    phi = 0.3
+    N = 12
+
+    def makePairAtoms(self):
+        for args in zip(self.atoms, *self.atom_args):
+            self.interaction.add(self.atom_type(*args))
+
+    def getMasses(self):
+        return [s ** 3 for s in self.getSigmas()]
+
+    def getSigmas(self):
+        N1 = self.N // 2
+        N2 = self.N - N1
+        return [1.] * N1 + [1.4] * N2
+
+    def setUp(self):
+        if capi.device_count() < 1:
+            self.fail("no GPU visible")
+        np.random.seed(self.seed)
+        radii = np.asarray(self.getSigmas()) / 2.
+        Vs = np.sum(radii ** 3) * 4 / 3 * np.pi
+        V = Vs / self.phi
+        self.L = float(V ** (1. / 3.))
+        self.box = d3.OriginBox(self.L)
+        self.atoms = d3.AtomVec(self.getMasses())
+        self.interaction = self.pair_type(self.box, self.atoms, 0.4)
+        self.makePairAtoms()
+        self.resetPositions()
+        collection_args = ([self.box, self.atoms] + list(self.collection_args) +
+                           [[self.interaction], [self.interaction.neighbor_list()], []])
+        self.collec = self.collection_type(*collection_args)
+
+    def resetPositions(self):
+        np.random.seed(131)
+        for a, m in zip(self.atoms, self.getMasses()):
+            a.x = np.random.uniform(0., self.L, size=(3,))
+            a.v = np.random.normal(size=(3,)) / m
+            a.f = np.random.normal(size=(3,))
+        nl = self.interaction.neighbor_list()
+        nl.update_list(True)
+
+    def reset(self):
+        self.resetPositions()
+        self.collec.scale_velocities_to_temp(1.0)
+        for _ in range(1000):
+            self.collec.timestep()
+            self.collec.scale_velocities_to_temp(1.0)
+
+
+class RandomHertzianVerletTest(PairedCollectionTest):
+    seed = 131
+    collection_type = d3.CollectionVerlet
+    pair_type = d3.Repulsion
+    atom_type = d3.EpsSigExpAtom
+    dt = 0.01
+    collection_args = [dt]
+
+    @property
+    def atom_args(self):
+        return [1.2] * self.N, self.getSigmas(), [2.0] * self.N
+
+    def testEnergy(self):
+        self.reset()
+        collec = self.collec
+        EKUTs = []
+        lastE = collec.energy()
+        for _ in range(1000):
+            for _ in range(10):
+                collec.timestep()
+                self.assertClose(collec.energy(), lastE, rtol=1e-2)
+                lastE = collec.energy()
+            EKUTs.append((collec.energy(), collec.kinetic_energy(),
+                          collec.potential_energy(), collec.temp()))
+        EKUTs = np.asarray(EKUTs)
+        E, K, U, T = EKUTs.T
+        self.assertClose(np.mean(T), 1.0, rtol=1e-1)
+        self.assertClose(np.std(E) / np.mean(E), 0.0, atol=1e-2)
+
+
+def _lj_dropin(n, seed, prec=None):
+    rng = np.random.default_rng(seed)
+    L = (n / 0.8) ** (1 / 3)
+    m = int(np.ceil(n ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+    x = (g + 0.5) * (L / m) + rng.normal(scale=0.05, size=(n, 3))
+    v = rng.normal(size=(n, 3))
+    v -= v.mean(0)
+    box = d3.OriginBox(L)
+    atoms = d3.AtomVec([1.0] * n)
+    if prec is not None:
+        atoms.set_device_precision(prec)
+    lj = d3.LJAttractRepulse(box, atoms, 0.3)
+    atoms.set_positions(x)
+    atoms.set_velocities(v)
+    for a in atoms:
+        lj.add(d3.IEpsSigCutAtom(a, [1.0], 0, 1.0, 2.5))
+    return box, atoms, lj, x, v, L
+
+
+def test_dropin_lj_matches_oracle():
+    if capi.device_count() < 1:
+        pytest.fail("no GPU visible")
+    n = 1500
+    box, atoms, lj, x, v, L = _lj_dropin(n, 3)
+    nl = lj.neighbor_list()
+    collec = d3.CollectionVerlet(box, atoms, 0.002, [lj], [nl])
+    t = O.PairTable(O.LJ_ATTRACT_REPULSE, [[1.0]], [1.0], [2.5], np.zeros(n, np.int32))
+    ref = O.System(np.ones(n), [L] * 3, t, np.full(n, 2.5), 0.3)
+    ref.set_state(x=x, v=v, a=np.zeros_like(x))
+    ref.initialize()
+    assert abs(collec.potential_energy() - ref.potential_energy()) <= 1e-12 * abs(
+        ref.potential_energy())
+    for _ in range(25):
+        collec.timestep()
+    ref.verlet(0.002, 25)
+    xr, vr, ar, fr = ref.state()
+    assert np.abs(atoms.get_positions() - xr).max() < 1e-11
+    assert np.abs(atoms.get_forces() - fr).max() < 1e-9 * np.abs(fr).max()
+    assert nl.which() == ref.which()
+    first, last = ref.pairs()
+    assert nl.numpairs() == len(first)
+    got = [(p.first().n(), p.last().n()) for p in nl]
+    assert got == list(zip(first.tolist(), last.tolist()))
+    with pytest.raises(IndexError):
+        nl[nl.numpairs()]
+    assert abs(collec.kinetic_energy() - ref.kinetic_energy()) <= 1e-12 * ref.kinetic_energy()
+    assert abs(lj.pressure(box) - ref.virial()) <= 1e-11 * abs(ref.virial())
+    S = lj.stress(box)
+    assert np.abs(S - ref.stress()).max() <= 1e-11 * np.abs(ref.stress()).max()
+    # host edits between steps are uploaded before the next step
+    a0 = atoms[0]
+    a0.v = [0.0, 0.0, 0.0]
+    collec.timestep()
+    ref_state = ref.state()
+    ref_v = ref_state[1].copy()
+    ref_v[0] = 0
+    ref.set_state(v=ref_v)
+    ref.verlet(0.002, 1)
+    assert np.abs(atoms.get_positions() - ref.state()[0]).max() < 1e-11
+
+
+def test_dropin_timesteps_and_mixed_precision():
+    if capi.device_count() < 1:
+        pytest.fail("no GPU visible")
+    box, atoms, lj, x, v, L = _lj_dropin(3000, 4, prec=d3.MIXED)
+    collec = d3.CollectionVerlet(box, atoms, 0.002, [lj], [lj.neighbor_list()])
+    e0 = collec.energy()
+    collec.timesteps(200)
+    assert abs(collec.energy() - e0) < 1e-3 * abs(e0)
+
+
+def test_dropin_damped_2d_relaxation():
+    if capi.device_count() < 1:
+        pytest.fail("no GPU visible")
+    n = 2000
+    rng = np.random.default_rng(8)
+    sig = np.where(rng.random(n) < 0.5, 1.0, 1.4)
+    L = float(np.sqrt(np.sum(np.pi / 4 * sig ** 2) / 0.8))
+    box = d2.OriginBox(L)
+    atoms = d2.AtomVec([1.0] * n)
+    rep = d2.Repulsion(box, atoms, 0.2)
+    atoms.set_positions(rng.uniform(0, L, (n, 2)))
+    for a, s in zip(atoms, sig):
+        rep.add(d2.EpsSigExpAtom(a, 1.0, float(s), 2.0))
+    collec = d2.CollectionDamped(box, atoms, 0.05, 1.0, [rep], [rep.neighbor_list()])
+    u0 = rep.energy(box)
+    collec.timesteps(500)
+    assert rep.energy(box) < 0.5 * u0
+    assert rep.overlaps(box) <= rep.contacts(box)
+
+
+def test_cpp_ljatoms_example_runs():
+    if capi.device_count() < 1:
+        pytest.fail("no GPU visible")
+    exe = os.path.join(ROOT, "parm_amd", "host", "examples", "LJatoms")
+    r = subprocess.run([exe, "4000", "5", "100"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("step")]
+    E = [float(ln.split()[3]) for ln in lines]
+    E0 = float([ln for ln in r.stdout.splitlines() if ln.startswith("E0")][0].split()[1])
+    assert len(E) == 5
+    assert max(abs(e - E0) for e in E) < 1e-3 * abs(E0)
