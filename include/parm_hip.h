@@ -195,6 +195,64 @@ int parm_profile_read(parm_system *sys, int reset, double *step_ms,
                       int64_t *step_launches, double *rebuild_ms,
                       int64_t *rebuilds);
 
+/* ---------------------------------------------------------------- batches */
+/* A parm_batch is nsys independent systems, each what the reference builds
+ * with one AtomVec (src/box.hpp:441-479), one OriginBox (src/box.hpp:97-157),
+ * one NeighborList with all atoms added (src/trackers.hpp:157-214), one
+ * NListed pair interaction and one CollectionDamped / CollectionVerlet
+ * (src/collection.hpp:273-295, 360-374).  The reference runs such systems one
+ * after another; here every step covers all systems, each system keeps its
+ * own box, list, skin test and rebuild count, and a run of steps needs no host
+ * decision (it is replayed from a HIP graph).
+ *
+ * Atoms are concatenated system after system: system s owns atoms
+ * [sum(natoms[0..s-1]), +natoms[s]).  Arrays are natoms_total x ndim.
+ * Limits: at most 4096 list cells per system (larger boxes use larger cells),
+ * list capacity fixed at parm_batch_pair_* time from the densest system; a run
+ * that overflows it fails with PARM_E_STATE. */
+typedef struct parm_batch parm_batch;
+
+/* L: nsys x ndim box edges (OriginBox(Vec) per system, src/box.hpp:102);
+ * skin: the NeighborList skin shared by all systems (src/trackers.cpp:10-17).
+ * Masses start at 1; positions, velocities, accelerations at 0. */
+int parm_batch_create(int ndim, int precision, int64_t nsys, const int64_t *natoms,
+                      const double *L, double skin, parm_batch **out);
+int parm_batch_destroy(parm_batch *b);
+int parm_batch_info(const parm_batch *b, int *ndim, int *nsys, int64_t *natoms_total);
+/* AtomVec masses (src/box.hpp:455-464); natoms_total values. */
+int parm_batch_set_masses(parm_batch *b, const double *masses);
+/* Atom.x / v / a / f of every atom (src/box.hpp:234-249); NULL skips. */
+int parm_batch_set_state(parm_batch *b, const double *x, const double *v, const double *a);
+int parm_batch_get_state(parm_batch *b, double *x, double *v, double *a, double *f);
+/* NListed<EpsSigExpAtom, RepulsionPair>::add / NListed<IEpsSigCutAtom,
+ * LJAttractRepulsePair>::add for every atom (src/interaction.hpp:1906-1910),
+ * parameters per type as in parm_pair_repulsion / parm_pair_lj_attract_repulse;
+ * atom_type has natoms_total entries.  Invalidates the lists. */
+int parm_batch_pair_repulsion(parm_batch *b, int ntypes, const double *eps,
+                              const double *sigma, const double *exponent,
+                              const int32_t *atom_type);
+int parm_batch_pair_lj_attract_repulse(parm_batch *b, int ntypes, const double *eps_rows,
+                                       const double *sigma, const double *sigcut,
+                                       const int32_t *atom_type);
+/* Collection::initialize of every system (src/collection.cpp:13-19). */
+int parm_batch_initialize(parm_batch *b);
+/* nsteps of CollectionDamped::timestep (src/collection.cpp:342-381) /
+ * CollectionVerlet::timestep (src/collection.cpp:442-469) on every system.
+ * use_graph != 0 replays the steps from a captured HIP graph (same results). */
+int parm_batch_damped_steps(parm_batch *b, double dt, double damping, int64_t nsteps,
+                            int use_graph);
+int parm_batch_verlet_steps(parm_batch *b, double dt, int64_t nsteps, int use_graph);
+/* NListed::energy and NListed::contacts per system (src/interaction.hpp:
+ * 2125-2150, 2255-2291); nsys values each, NULL skips. */
+int parm_batch_sums(parm_batch *b, double *U, double *contacts);
+/* NeighborList::which() and numpairs() per system (src/trackers.hpp:187-212). */
+int parm_batch_lists(parm_batch *b, uint32_t *which, int64_t *npairs);
+/* The pairs of system sysid's list as (first, last) atom indices within the
+ * system, first > last, sorted by first then last.  Writes min(cap, n) pairs
+ * and the total n. */
+int parm_batch_pairs(parm_batch *b, int64_t sysid, int64_t cap, int32_t *first,
+                     int32_t *last, int64_t *npairs);
+
 #ifdef __cplusplus
 }
 #endif

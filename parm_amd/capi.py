@@ -32,6 +32,11 @@ EXPORTS = (
     "parm_force_stats", "parm_initialize", "parm_verlet_steps",
     "parm_damped_steps", "parm_synchronize", "parm_profile_enable",
     "parm_profile_read",
+    "parm_batch_create", "parm_batch_destroy", "parm_batch_info",
+    "parm_batch_set_masses", "parm_batch_set_state", "parm_batch_get_state",
+    "parm_batch_pair_repulsion", "parm_batch_pair_lj_attract_repulse",
+    "parm_batch_initialize", "parm_batch_damped_steps", "parm_batch_verlet_steps",
+    "parm_batch_sums", "parm_batch_lists", "parm_batch_pairs",
 )
 
 
@@ -96,6 +101,22 @@ def lib():
             "parm_profile_enable": (i32, [_vp, i32]),
             "parm_profile_read": (i32, [_vp, i32, _dp, C.POINTER(C.c_int64), _dp,
                                         C.POINTER(C.c_int64)]),
+            "parm_batch_create": (i32, [i32, i32, i64, C.POINTER(C.c_int64), _dp, dbl,
+                                        C.POINTER(_vp)]),
+            "parm_batch_destroy": (i32, [_vp]),
+            "parm_batch_info": (i32, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int64)]),
+            "parm_batch_set_masses": (i32, [_vp, _dp]),
+            "parm_batch_set_state": (i32, [_vp, _dp, _dp, _dp]),
+            "parm_batch_get_state": (i32, [_vp, _dp, _dp, _dp, _dp]),
+            "parm_batch_pair_repulsion": (i32, [_vp, i32, _dp, _dp, _dp, _ip]),
+            "parm_batch_pair_lj_attract_repulse": (i32, [_vp, i32, _dp, _dp, _dp, _ip]),
+            "parm_batch_initialize": (i32, [_vp]),
+            "parm_batch_damped_steps": (i32, [_vp, dbl, dbl, i64, i32]),
+            "parm_batch_verlet_steps": (i32, [_vp, dbl, i64, i32]),
+            "parm_batch_sums": (i32, [_vp, _dp, _dp]),
+            "parm_batch_lists": (i32, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]),
+            "parm_batch_pairs": (i32, [_vp, i64, i64, _ip, _ip, C.POINTER(C.c_int64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -280,3 +301,94 @@ class System:
                                        C.byref(rm), C.byref(rn)))
         return dict(step_ms=sm.value, step_launches=sn.value, rebuild_ms=rm.value,
                     rebuilds=rn.value)
+
+
+class Batch:
+    """Many independent systems stepped together (parm_batch_* in
+    include/parm_hip.h).  natoms[s] atoms of system s, concatenated in system
+    order; L is nsys x ndim box edges."""
+
+    def __init__(self, ndim, natoms, L, skin, precision=FP64):
+        self.ndim = int(ndim)
+        self.natoms = np.ascontiguousarray(natoms, dtype=np.int64).ravel()
+        self.nsys = self.natoms.size
+        self.offsets = np.concatenate([[0], np.cumsum(self.natoms)]).astype(np.int64)
+        self.n = int(self.offsets[-1])
+        self.precision = precision
+        L = _f64(np.broadcast_to(np.asarray(L, float).reshape(self.nsys, -1),
+                                 (self.nsys, self.ndim)))
+        h = _vp()
+        _check(lib().parm_batch_create(self.ndim, int(precision), self.nsys,
+                                       self.natoms.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       _d(L), float(skin), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().parm_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_masses(self, m):
+        _check(lib().parm_batch_set_masses(self.h, _d(_f64(m).ravel())))
+
+    def set_state(self, x=None, v=None, a=None):
+        sh = (self.n, self.ndim)
+        arrs = [None if q is None else _f64(q, sh) for q in (x, v, a)]
+        _check(lib().parm_batch_set_state(self.h, *[_d(q) for q in arrs]))
+
+    def state(self):
+        sh = (self.n, self.ndim)
+        x, v, a, f = (np.empty(sh) for _ in range(4))
+        _check(lib().parm_batch_get_state(self.h, _d(x), _d(v), _d(a), _d(f)))
+        return x, v, a, f
+
+    def set_repulsion(self, eps, sigma, exponent, types):
+        sigma = _f64(sigma).ravel()
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        _check(lib().parm_batch_pair_repulsion(self.h, sigma.size, _d(_f64(eps).ravel()),
+                                               _d(sigma), _d(_f64(exponent).ravel()),
+                                               t.ctypes.data_as(_ip)))
+
+    def set_lj(self, eps_rows, sigma, sigcut, types):
+        sigma = _f64(sigma).ravel()
+        nt = sigma.size
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        _check(lib().parm_batch_pair_lj_attract_repulse(
+            self.h, nt, _d(_f64(eps_rows, (nt, nt))), _d(sigma), _d(_f64(sigcut).ravel()),
+            t.ctypes.data_as(_ip)))
+
+    def initialize(self):
+        _check(lib().parm_batch_initialize(self.h))
+
+    def damped(self, dt, damping, nsteps=1, graph=True):
+        _check(lib().parm_batch_damped_steps(self.h, float(dt), float(damping), int(nsteps),
+                                             int(bool(graph))))
+
+    def verlet(self, dt, nsteps=1, graph=True):
+        _check(lib().parm_batch_verlet_steps(self.h, float(dt), int(nsteps),
+                                             int(bool(graph))))
+
+    def sums(self):
+        U = np.empty(self.nsys)
+        c = np.empty(self.nsys)
+        _check(lib().parm_batch_sums(self.h, _d(U), _d(c)))
+        return U, c
+
+    def lists(self):
+        w = np.empty(self.nsys, np.uint32)
+        n = np.empty(self.nsys, np.int64)
+        _check(lib().parm_batch_lists(self.h, w.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                      n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return w, n
+
+    def pairs(self, s):
+        got = C.c_int64(0)
+        _check(lib().parm_batch_pairs(self.h, int(s), 0, None, None, C.byref(got)))
+        n = got.value
+        first = np.empty(max(n, 1), np.int32)
+        last = np.empty(max(n, 1), np.int32)
+        _check(lib().parm_batch_pairs(self.h, int(s), n, first.ctypes.data_as(_ip),
+                                      last.ctypes.data_as(_ip), C.byref(got)))
+        return first[:n], last[:n]

@@ -17,17 +17,24 @@
 #include "../../include/parm_hip.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "pairtable.hpp"
 
 using namespace parm;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string &msg) {
+namespace parm {
+int set_error(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+}  // namespace parm
+
+namespace {
+
+int fail(int code, const std::string &msg) { return set_error(code, msg); }
 
 #define PARM_HIP(call)                                                         \
     do {                                                                       \
@@ -84,7 +91,7 @@ struct parm_system {
     bool box_set = false;
 
     // per-slot device arrays (slot order = spatial sort order)
-    void *pos = nullptr, *pos_alt = nullptr, *xref = nullptr;
+    void *pos = nullptr, *pos_alt = nullptr, *xref = nullptr, *xwrap = nullptr;
     double *v[3] = {nullptr, nullptr, nullptr};
     double *a[3] = {nullptr, nullptr, nullptr};
     double *f[3] = {nullptr, nullptr, nullptr};
@@ -174,6 +181,7 @@ struct parm_system {
     ReduceBuf redbuf(int flag_index) const {
         ReduceBuf R{};
         R.part = part;
+        R.pstride = force_blocks(std::max(n, 1), 8);
         R.out = out;
         R.flag = dflag + flag_index;
         R.skin = skin;
@@ -305,10 +313,11 @@ int rebuild(parm_system *s) {
     for (int attempt = 0; attempt < 4; attempt++) {
         PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
         PARM_HIP(hipMemsetAsync(s->dflag + kSlotBuild, 0, sizeof(uint32_t), st));
-        PARM_HIP(launch_build_list(s->ndim, n, s->pos, s->rank, s->diam, s->type, s->keys,
-                                   s->cstart, s->grid, s->box, s->skin, s->type_bits,
-                                   s->tpa, s->cap, s->npad, s->nbr, s->ncount, s->xref,
-                                   s->bpart, s->stats, s->dflag + kSlotBuild, st));
+        PARM_HIP(launch_build_list(s->ndim, n, s->pos, s->xwrap, s->rank, s->diam, s->type,
+                                   s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
+                                   s->type_bits, s->tpa, s->cap, s->npad, s->nbr,
+                                   s->ncount, s->xref, s->bpart, s->stats,
+                                   s->dflag + kSlotBuild, st));
         unsigned long long hs[2] = {0, 0};
         PARM_HIP(hipMemcpyAsync(hs, s->stats, sizeof(hs), hipMemcpyDeviceToHost, st));
         PARM_HIP(hipStreamSynchronize(st));
@@ -400,10 +409,10 @@ int relayout(parm_system *s) {
     if (!s->built || s->n == 0) return PARM_OK;
     hipStream_t st = s->stream;
     PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
-    PARM_HIP(launch_build_list(s->ndim, s->n, s->xref, s->rank, s->diam, s->type, s->keys,
-                               s->cstart, s->grid, s->box, s->skin, s->type_bits, s->tpa,
-                               s->cap, s->npad, s->nbr, s->ncount, s->xref, s->bpart,
-                               s->stats, s->dflag + kSlotBuild, st));
+    PARM_HIP(launch_build_list(s->ndim, s->n, s->xref, s->xwrap, s->rank, s->diam, s->type,
+                               s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
+                               s->type_bits, s->tpa, s->cap, s->npad, s->nbr, s->ncount,
+                               s->xref, s->bpart, s->stats, s->dflag + kSlotBuild, st));
     return PARM_OK;
 }
 
@@ -564,6 +573,7 @@ void destroy(parm_system *s) {
     dfree(s->pos);
     dfree(s->pos_alt);
     dfree(s->xref);
+    dfree(s->xwrap);
     for (int k = 0; k < 3; k++) {
         dfree(s->v[k]);
         dfree(s->a[k]);
@@ -621,59 +631,9 @@ int build_pair_table(parm_system *s, int kind, int ntypes, const double *eps,
     for (int i = 0; i < s->n; i++)
         if (atype[i] < 0 || atype[i] >= ntypes)
             return fail(PARM_E_INVALID, "atom type out of range at atom " + std::to_string(i));
-    std::vector<PairParam> tab((size_t)ntypes * ntypes);
+    std::vector<PairParam> tab;
     bool harm = true;
-    for (int t1 = 0; t1 < ntypes; t1++) {
-        for (int t2 = 0; t2 < ntypes; t2++) {
-            PairParam p{};
-            if (kind == kPairLJ) {
-                // LJAttractRepulsePair(a1, a2)  (interaction.hpp:1256-1269)
-                double e = eps[(size_t)t1 * ntypes + t2];
-                double sig = (sigma[t1] + sigma[t2]) / 2.0;
-                double cut = std::max(third[t1], third[t2]);
-                double cut_energy;
-                if (e <= 0) {
-                    cut = 1;
-                    cut_energy = 0;
-                    e = std::fabs(e);
-                } else {
-                    double mid = (1 - std::pow(cut, -6));
-                    cut_energy = e * (mid * mid);
-                }
-                p.eps = e;
-                p.sig = sig;
-                p.sig2 = sig * sig;
-                p.cut2 = cut * cut;
-                p.lim_lo = p.sig2 * p.cut2 * (1.0 - 1e-14);
-                p.lim_hi = p.sig2 * p.cut2 * (1.0 + 1e-14);
-                p.p5 = cut_energy;
-                p.p6 = 12 * e;
-                p.lim_f = (float)(p.sig2 * p.cut2);
-            } else {
-                // RepulsionPair(a1, a2)  (interaction.hpp:1532-1537)
-                double e = std::sqrt(eps[t1] * eps[t2]);
-                double sig = (sigma[t1] + sigma[t2]) / 2.0;
-                double nexp = (third[t1] + third[t2]) / 2.0;
-                p.eps = e;
-                p.sig = sig;
-                p.sig2 = sig * sig;
-                p.cut2 = p.sig2;
-                p.lim_lo = p.sig2;
-                p.lim_hi = p.sig2;
-                p.p5 = nexp;
-                p.p6 = nexp - 1;
-                p.lim_f = (float)p.sig2;
-                if (nexp != 2.0) harm = false;
-            }
-            p.eps_f = (float)p.eps;
-            p.sig2_f = (float)p.sig2;
-            p.p5_f = (float)p.p5;
-            p.p6_f = (float)p.p6;
-            p.sig_f = (float)p.sig;
-            p.isig_f = (float)(1.0 / p.sig);
-            tab[(size_t)t1 * ntypes + t2] = p;
-        }
-    }
+    make_pair_table(kind, ntypes, eps, sigma, third, tab, harm);
     PARM_CHECK(set_device(s));
     dfree(s->tab);
     PARM_CHECK(dalloc(&s->tab, tab.size()));
@@ -769,7 +729,8 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
     const size_t n = (size_t)std::max(1, s->n);
     if ((rc = dalloc((char **)&s->pos, n * s->pos_bytes)) ||
         (rc = dalloc((char **)&s->pos_alt, n * s->pos_bytes)) ||
-        (rc = dalloc((char **)&s->xref, n * s->pos_bytes)))
+        (rc = dalloc((char **)&s->xref, n * s->pos_bytes)) ||
+        (rc = dalloc((char **)&s->xwrap, n * s->pos_bytes)))
         return bail(rc);
     for (int k = 0; k < 3; k++) {
         if ((rc = dalloc(&s->v[k], n)) || (rc = dalloc(&s->a[k], n)) ||

@@ -1,0 +1,529 @@
+// batch.hip -- many independent systems (packings) stepped together.
+//
+// Each system is what the reference builds with one AtomVec, OriginBox,
+// NeighborList, NListed pair interaction and CollectionDamped /
+// CollectionVerlet (src/collection.cpp:356-381, 442-469; src/trackers.cpp:
+// 19-85).  The systems' atoms are concatenated; every per-step kernel covers
+// all systems at once, and every decision (skin test, rebuild) is taken on
+// the device per system, so a step needs no host round trip and a run of
+// steps can be captured in a HIP graph.
+//
+// Per step:  k_batch_rebuild one workgroup per system; exits unless that
+//                            system's skin flag is set: cell sort in LDS,
+//                            exact list build, xref = x
+//            k_batch_force   forces + kick (+ next drift, |x - xref| per atom)
+//            k_batch_skin    one wavefront per system: top-2 of |x - xref|
+//                            -> the system's flag for the next rebuild
+#include "common.hpp"
+#include "kernels.hpp"
+#include "pairs.hpp"
+
+namespace parm {
+
+// ---------------------------------------------------------------- forces
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, bool ENERGY,
+          int TPA>
+__global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView Lv,
+                                                        PairView P, StepCoef c,
+                                                        double *__restrict__ eatom) {
+    using PT = typename PosStore<D>::T;
+    constexpr bool DRIFT =
+        (STAGE == kStageVerletKickDrift || STAGE == kStageDampedKickDrift);
+    constexpr bool DAMPED =
+        (STAGE == kStageDampedKick || STAGE == kStageDampedKickDrift);
+    const int gt = blockIdx.x * kBlock + threadIdx.x;
+    const int i = gt / TPA;
+    const int sub = gt % TPA;
+    const bool act = i < A.ntot;
+    PairAcc acc;
+    double fi[D];
+    float f32[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        fi[k] = 0.0;
+        f32[k] = 0.0f;
+    }
+    const PT *pos = static_cast<const PT *>(A.pos_in);
+    Vec<D> xi;
+    Box box{};
+    if (act) {
+        xi = PosStore<D>::load(pos[i]);
+        box = A.geom[A.sys[i]].box;
+    }
+    if (act && Lv.built) {
+        const int cnt = Lv.ncount[i];
+        const size_t stride = (size_t)Lv.npad * TPA;
+        const uint32_t *nb = Lv.nbr + (size_t)i * TPA + sub;
+        const int tb = Lv.type_bits;
+        const uint32_t tmask = (1u << tb) - 1u;
+        const PairParam *row = P.tab;
+        PairParam p0;
+        if (SINGLE)
+            p0 = P.tab[0];
+        else
+            row = P.tab + (size_t)A.type[i] * P.ntypes;
+        int q = sub;
+        size_t off = 0;
+        for (; q + TPA < cnt; q += 2 * TPA, off += 2 * stride) {
+            const uint32_t e0 = nb[off];
+            const uint32_t e1 = nb[off + stride];
+            const PT pj0 = pos[e0 >> tb];
+            const PT pj1 = pos[e1 >> tb];
+            const Vec<D> r0 = diff<D>(xi, PosStore<D>::load(pj0), box);
+            visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e0 & tmask], r0, fi, f32,
+                                               acc);
+            const Vec<D> r1 = diff<D>(xi, PosStore<D>::load(pj1), box);
+            visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e1 & tmask], r1, fi, f32,
+                                               acc);
+        }
+        if (q < cnt) {
+            const uint32_t e0 = nb[off];
+            const Vec<D> r0 = diff<D>(xi, PosStore<D>::load(pos[e0 >> tb]), box);
+            visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e0 & tmask], r0, fi, f32,
+                                               acc);
+        }
+    }
+    if (PREC != kFP64) {
+#pragma unroll
+        for (int k = 0; k < D; k++) fi[k] = (double)f32[k];
+    }
+    double e = acc.e, nc = acc.contacts;
+#pragma unroll
+    for (int o = 1; o < TPA; o <<= 1) {
+#pragma unroll
+        for (int k = 0; k < D; k++) fi[k] += __shfl_xor(fi[k], o);
+        if (ENERGY) {
+            e += __shfl_xor(e, o);
+            nc += __shfl_xor(nc, o);
+        }
+    }
+    if (!act || sub != 0) return;
+    if (ENERGY) {
+        eatom[2 * (size_t)i] = 0.5 * e;
+        eatom[2 * (size_t)i + 1] = nc;
+    }
+    if (STAGE == kStageSums) return;
+#pragma unroll
+    for (int k = 0; k < D; k++) A.f[k][i] = fi[k];
+    if (STAGE == kStageForceOnly) return;
+    const double m = A.m[i];
+    const bool fixed = is_fixed(m);
+    double a[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) a[k] = fixed ? 0.0 : fi[k] / m;
+#pragma unroll
+    for (int k = 0; k < D; k++) A.a[k][i] = a[k];
+    if (STAGE == kStageForceA) return;
+    double v[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) v[k] = A.v[k][i];
+    if (!fixed) {
+#pragma unroll
+        for (int k = 0; k < D; k++) v[k] = v[k] + a[k] * (DAMPED ? c.k4 : c.k2);
+    }
+    if (DRIFT) {
+        Vec<D> xn = xi;
+        drift_atom<D, DAMPED>(xn, v, a, fixed, c);
+        static_cast<PT *>(A.pos_out)[i] = PosStore<D>::make(xn);
+        const Vec<D> x0 = PosStore<D>::load(static_cast<const PT *>(A.xref)[i]);
+        Vec<D> d;
+#pragma unroll
+        for (int k = 0; k < D; k++) d.c[k] = xn.c[k] - x0.c[k];
+        A.disp[i] = sqrt(sqnorm<D>(d));
+    }
+#pragma unroll
+    for (int k = 0; k < D; k++) A.v[k][i] = v[k];
+}
+
+// First half of a step (drift + half kick) and |x - xref|.
+template <int D, bool DAMPED>
+__global__ __launch_bounds__(kBlock) void k_batch_drift(BatchArrays A, StepCoef c) {
+    using PT = typename PosStore<D>::T;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= A.ntot) return;
+    PT *pos = static_cast<PT *>(A.pos_out);
+    Vec<D> x = PosStore<D>::load(pos[i]);
+    double v[D], a[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        v[k] = A.v[k][i];
+        a[k] = A.a[k][i];
+    }
+    const bool fixed = is_fixed(A.m[i]);
+    drift_atom<D, DAMPED>(x, v, a, fixed, c);
+    if (!fixed) pos[i] = PosStore<D>::make(x);
+#pragma unroll
+    for (int k = 0; k < D; k++) A.v[k][i] = v[k];
+    const Vec<D> x0 = PosStore<D>::load(static_cast<const PT *>(A.xref)[i]);
+    Vec<D> d;
+#pragma unroll
+    for (int k = 0; k < D; k++) d.c[k] = x.c[k] - x0.c[k];
+    A.disp[i] = sqrt(sqnorm<D>(d));
+}
+
+// |x - xref| for every atom (update_list(false) outside a step).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_batch_disp(BatchArrays A) {
+    using PT = typename PosStore<D>::T;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= A.ntot) return;
+    const Vec<D> x = PosStore<D>::load(static_cast<const PT *>(A.pos_in)[i]);
+    const Vec<D> x0 = PosStore<D>::load(static_cast<const PT *>(A.xref)[i]);
+    Vec<D> d;
+#pragma unroll
+    for (int k = 0; k < D; k++) d.c[k] = x.c[k] - x0.c[k];
+    A.disp[i] = sqrt(sqnorm<D>(d));
+}
+
+// ------------------------------------------------------------ skin test
+// One wavefront per system: the reference's rule (src/trackers.cpp:23-53),
+// rebuild iff the two largest displacements sum to >= skin, checked only
+// once something moved.  The flag is overwritten, never accumulated.
+__global__ __launch_bounds__(kBlock) void k_batch_skin(int nsys, BatchArrays A,
+                                                       double skin,
+                                                       uint32_t *__restrict__ flags,
+                                                       uint32_t *__restrict__ gflag) {
+    const int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nsys) return;
+    const int off = A.geom[s].off, n = A.geom[s].n;
+    Top2 t;
+    bool bad = false;
+    for (int k = lane; k < n; k += kWave) {
+        const double d = A.disp[off + k];
+        t.add(d);
+        bad |= !isfinite(d);
+    }
+    t = wave_top2(t);
+    const unsigned long long anybad = __ballot(bad);
+    if (lane == 0) {
+        flags[s] = (t.a > 0 && t.b + t.a >= skin) ? 1u : 0u;
+        if (anybad) atomicOr(gflag, kFlagNonFinite);
+    }
+}
+
+// -------------------------------------------------------------- rebuild
+// One workgroup per system.  Exits unless the system's flag is set (or
+// force).  Cell keys and a stable counting sort of the system's atoms by
+// cell in LDS, then the exact list build over the periodic 27-cell
+// stencil (candidates in cell order, then index order: deterministic), then
+// xref = x.
+constexpr int kBatchMaxCells = kBatchMaxCellsPerSystem;
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_batch_rebuild(
+    BatchArrays A, const typename PosStore<D>::T *__restrict__ pos, int force, int count_rebuild,
+    double skin, int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
+    int32_t *__restrict__ ncount, int32_t *__restrict__ keys,
+    int32_t *__restrict__ cellidx, int32_t *__restrict__ cstart,
+    uint32_t *__restrict__ flags, uint32_t *__restrict__ which,
+    unsigned long long *__restrict__ npairs, uint32_t *__restrict__ gflag) {
+    using PT = typename PosStore<D>::T;
+    const int s = blockIdx.x;
+    if (!force && flags[s] == 0u) return;
+    __shared__ int hist[kBatchMaxCells];
+    __shared__ int ck[kBlock];
+    __shared__ unsigned long long red[kBlock / kWave];
+    const BatchGeom G = A.geom[s];
+    const int off = G.off, n = G.n, nc0 = G.grid.nc[0], nc1 = G.grid.nc[1],
+              nc2 = G.grid.nc[2], ncell = G.grid.ncell;
+    const int t = threadIdx.x;
+    for (int c = t; c < ncell; c += kBlock) hist[c] = 0;
+    __syncthreads();
+    // keys and counts
+    for (int k = t; k < n; k += kBlock) {
+        const Vec<D> x = PosStore<D>::load(pos[off + k]);
+        int key = 0;
+#pragma unroll
+        for (int d = D - 1; d >= 0; d--) {
+            double sv = x.c[d] * G.grid.invL[d];
+            sv -= floor(sv);
+            int cd = (int)(sv * (double)G.grid.nc[d]);
+            cd = cd < 0 ? 0 : (cd >= G.grid.nc[d] ? G.grid.nc[d] - 1 : cd);
+            key = key * G.grid.nc[d] + cd;
+        }
+        keys[off + k] = key;
+        atomicAdd(&hist[key], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts (thread 0; at most kBatchMaxCells cells)
+    if (t == 0) {
+        int run = 0;
+        for (int c = 0; c < ncell; c++) {
+            const int h = hist[c];
+            cstart[G.cell_off + c] = off + run;
+            hist[c] = run;
+            run += h;
+        }
+        cstart[G.cell_off + ncell] = off + run;
+    }
+    __syncthreads();
+    // stable scatter, one chunk of kBlock atoms at a time
+    for (int base = 0; base < n; base += kBlock) {
+        const int k = base + t;
+        const int key = k < n ? keys[off + k] : -1;
+        ck[t] = key;
+        __syncthreads();
+        if (k < n) {
+            int r = 0;
+            for (int u = 0; u < t; u++) r += (ck[u] == key);
+            cellidx[off + hist[key] + r] = off + k;
+        }
+        __syncthreads();
+        if (k < n) atomicAdd(&hist[key], 1);
+        __syncthreads();
+    }
+    // xref = x (the reference stores lastlocs inside the rebuild loop)
+    for (int k = t; k < n; k += kBlock) static_cast<PT *>(A.xref_w)[off + k] = pos[off + k];
+    __threadfence_block();
+    __syncthreads();
+    // list build
+    unsigned long long mycnt = 0;
+    for (int k = t; k < n; k += kBlock) {
+        const int i = off + k;
+        const Vec<D> xi = PosStore<D>::load(pos[i]);
+        const double di = A.diam[i];
+        int key = keys[i];
+        int ci[3] = {0, 0, 0};
+        ci[0] = key % nc0;
+        key /= nc0;
+        if (D >= 2) {
+            ci[1] = key % nc1;
+            key /= nc1;
+        }
+        if (D == 3) ci[2] = key % nc2;
+        const int lo0 = nc0 >= 3 ? -1 : 0, lo1 = (D >= 2 && nc1 >= 3) ? -1 : 0,
+                  lo2 = (D == 3 && nc2 >= 3) ? -1 : 0;
+        int cnt = 0;
+        for (int oz = lo2; oz <= -lo2; oz++)
+            for (int oy = lo1; oy <= -lo1; oy++)
+                for (int ox = lo0; ox <= -lo0; ox++) {
+                    int q0 = ci[0] + ox, q1 = ci[1] + oy, q2 = ci[2] + oz;
+                    q0 = q0 < 0 ? q0 + nc0 : (q0 >= nc0 ? q0 - nc0 : q0);
+                    q1 = q1 < 0 ? q1 + nc1 : (q1 >= nc1 ? q1 - nc1 : q1);
+                    q2 = q2 < 0 ? q2 + nc2 : (q2 >= nc2 ? q2 - nc2 : q2);
+                    const int cc = (D == 3 ? (q2 * nc1 + q1) * nc0 : q1 * nc0) + q0;
+                    const int s0 = cstart[G.cell_off + cc], s1 = cstart[G.cell_off + cc + 1];
+                    for (int u = s0; u < s1; u++) {
+                        const int j = cellidx[u];
+                        if (j == i) continue;
+                        if (in_list<D>(xi, PosStore<D>::load(pos[j]), di, A.diam[j], G.box,
+                                       skin)) {
+                            if (cnt < cap)
+                                nbr[((size_t)(cnt / tpa) * npad + i) * tpa + cnt % tpa] =
+                                    ((uint32_t)j << type_bits) | (uint32_t)A.type[j];
+                            cnt++;
+                        }
+                    }
+                }
+        ncount[i] = cnt < cap ? cnt : cap;
+        if (cnt > cap) atomicOr(gflag, kFlagOverflow);
+        mycnt += (unsigned long long)cnt;
+    }
+    unsigned long long sum = mycnt;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if (t % kWave == 0) red[t / kWave] = sum;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long tot = 0;
+        for (int q = 0; q < kBlock / kWave; q++) tot += red[q];
+        npairs[s] = tot / 2;
+        if (count_rebuild) which[s] += 1u;
+        flags[s] = 0u;
+    }
+}
+
+// Per-system sums of the per-atom energies / contact counts.
+__global__ __launch_bounds__(kBlock) void k_batch_sys_sums(int nsys, BatchArrays A,
+                                                           const double *__restrict__ eatom,
+                                                           double *__restrict__ U,
+                                                           double *__restrict__ contacts) {
+    const int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nsys) return;
+    const int off = A.geom[s].off, n = A.geom[s].n;
+    KSum e;
+    double c = 0.0;
+    for (int k = lane; k < n; k += kWave) {
+        e.add(eatom[2 * (size_t)(off + k)]);
+        c += eatom[2 * (size_t)(off + k) + 1];
+    }
+    CSum cs{e.s, e.c};
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        cs = csum_merge(cs, CSum{__shfl_xor(cs.s, o), __shfl_xor(cs.c, o)});
+        c += __shfl_xor(c, o);
+    }
+    if (lane == 0) {
+        U[s] = cs.s + cs.c;
+        contacts[s] = c / 2;
+    }
+}
+
+// ======================================================== host launchers
+static inline int nblk1(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, int TPA>
+static void bforce_tt(bool energy, const BatchArrays &A, const ListView &Lv,
+                      const PairView &P, const StepCoef &c, double *eatom, hipStream_t s) {
+    // energies are accumulated by the sums stage only
+    (void)energy;
+    dim3 g(nblk1((long)A.ntot * TPA)), b(kBlock);
+    hipLaunchKernelGGL(
+        (k_batch_force<D, KIND, PREC, SINGLE, HARM, STAGE, STAGE == kStageSums, TPA>), g, b,
+        0, s, A, Lv, P, c, eatom);
+}
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE>
+static void bforce_t(bool energy, const BatchArrays &A, const ListView &Lv,
+                     const PairView &P, const StepCoef &c, double *eatom, hipStream_t s) {
+    switch (Lv.tpa) {
+    case 1:
+        bforce_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 1>(energy, A, Lv, P, c, eatom, s);
+        break;
+    case 2:
+        bforce_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 2>(energy, A, Lv, P, c, eatom, s);
+        break;
+    case 8:
+        bforce_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 8>(energy, A, Lv, P, c, eatom, s);
+        break;
+    default:
+        bforce_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 4>(energy, A, Lv, P, c, eatom, s);
+        break;
+    }
+}
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM>
+static void bforce_s(int stage, bool energy, const BatchArrays &A, const ListView &Lv,
+                     const PairView &P, const StepCoef &c, double *eatom, hipStream_t s) {
+    switch (stage) {
+#define PARM_BSTAGE(S)                                                          \
+    case S:                                                                     \
+        bforce_t<D, KIND, PREC, SINGLE, HARM, S>(energy, A, Lv, P, c, eatom, s); \
+        break;
+        PARM_BSTAGE(kStageForceOnly)
+        PARM_BSTAGE(kStageForceA)
+        PARM_BSTAGE(kStageVerletKick)
+        PARM_BSTAGE(kStageVerletKickDrift)
+        PARM_BSTAGE(kStageDampedKick)
+        PARM_BSTAGE(kStageDampedKickDrift)
+        PARM_BSTAGE(kStageSums)
+#undef PARM_BSTAGE
+    }
+}
+
+template <int D, int KIND, int PREC>
+static void bforce_k(int stage, bool energy, const BatchArrays &A, const ListView &Lv,
+                     const PairView &P, const StepCoef &c, double *eatom, hipStream_t s) {
+    if constexpr (KIND == kPairRepulsion) {
+        if (P.harmonic) {
+            if (P.single)
+                bforce_s<D, KIND, PREC, true, true>(stage, energy, A, Lv, P, c, eatom, s);
+            else
+                bforce_s<D, KIND, PREC, false, true>(stage, energy, A, Lv, P, c, eatom, s);
+            return;
+        }
+    }
+    {
+        if (P.single)
+            bforce_s<D, KIND, PREC, true, false>(stage, energy, A, Lv, P, c, eatom, s);
+        else
+            bforce_s<D, KIND, PREC, false, false>(stage, energy, A, Lv, P, c, eatom, s);
+    }
+}
+
+template <int D>
+static void bforce_d(int prec, int stage, bool energy, const BatchArrays &A,
+                     const ListView &Lv, const PairView &P, const StepCoef &c,
+                     double *eatom, hipStream_t s) {
+    if (P.kind == kPairRepulsion) {
+        if (prec == kFP64)
+            bforce_k<D, kPairRepulsion, kFP64>(stage, energy, A, Lv, P, c, eatom, s);
+        else
+            bforce_k<D, kPairRepulsion, kMixed>(stage, energy, A, Lv, P, c, eatom, s);
+    } else {
+        if (prec == kFP64)
+            bforce_k<D, kPairLJ, kFP64>(stage, energy, A, Lv, P, c, eatom, s);
+        else
+            bforce_k<D, kPairLJ, kMixed>(stage, energy, A, Lv, P, c, eatom, s);
+    }
+}
+
+hipError_t launch_batch_force(int ndim, int precision, int stage, bool energy,
+                              const BatchArrays &A, const ListView &Lv, const PairView &P,
+                              const StepCoef &c, double *eatom, hipStream_t s) {
+    if (A.ntot <= 0) return hipSuccess;
+    if (ndim == 3)
+        bforce_d<3>(precision, stage, energy, A, Lv, P, c, eatom, s);
+    else
+        bforce_d<2>(precision, stage, energy, A, Lv, P, c, eatom, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_drift(int ndim, bool damped, const BatchArrays &A,
+                              const StepCoef &c, hipStream_t s) {
+    if (A.ntot <= 0) return hipSuccess;
+    dim3 g(nblk1(A.ntot)), b(kBlock);
+    if (ndim == 3) {
+        if (damped)
+            hipLaunchKernelGGL((k_batch_drift<3, true>), g, b, 0, s, A, c);
+        else
+            hipLaunchKernelGGL((k_batch_drift<3, false>), g, b, 0, s, A, c);
+    } else {
+        if (damped)
+            hipLaunchKernelGGL((k_batch_drift<2, true>), g, b, 0, s, A, c);
+        else
+            hipLaunchKernelGGL((k_batch_drift<2, false>), g, b, 0, s, A, c);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_disp(int ndim, const BatchArrays &A, hipStream_t s) {
+    if (A.ntot <= 0) return hipSuccess;
+    dim3 g(nblk1(A.ntot)), b(kBlock);
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_batch_disp<3>, g, b, 0, s, A);
+    else
+        hipLaunchKernelGGL(k_batch_disp<2>, g, b, 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_skin(int nsys, const BatchArrays &A, double skin, uint32_t *flags,
+                             uint32_t *gflag, hipStream_t s) {
+    const int per = kBlock / kWave;
+    hipLaunchKernelGGL(k_batch_skin, dim3((nsys + per - 1) / per), dim3(kBlock), 0, s, nsys,
+                       A, skin, flags, gflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const void *pos,
+                                int force, int count_rebuild, double skin,
+                                int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
+                                int32_t *ncount, int32_t *keys, int32_t *cellidx,
+                                int32_t *cstart, uint32_t *flags, uint32_t *which,
+                                unsigned long long *npairs, uint32_t *gflag,
+                                hipStream_t s) {
+    if (nsys <= 0) return hipSuccess;
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_batch_rebuild<3>, dim3(nsys), dim3(kBlock), 0, s, A,
+                           (const double4 *)pos, force, count_rebuild,
+                           skin, type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx,
+                           cstart, flags, which, npairs, gflag);
+    else
+        hipLaunchKernelGGL(k_batch_rebuild<2>, dim3(nsys), dim3(kBlock), 0, s, A,
+                           (const double2 *)pos, force, count_rebuild,
+                           skin, type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx,
+                           cstart, flags, which, npairs, gflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_sys_sums(int nsys, const BatchArrays &A, const double *eatom,
+                                 double *U, double *contacts, hipStream_t s) {
+    const int per = kBlock / kWave;
+    hipLaunchKernelGGL(k_batch_sys_sums, dim3((nsys + per - 1) / per), dim3(kBlock), 0, s,
+                       nsys, A, eatom, U, contacts);
+    return hipGetLastError();
+}
+
+}  // namespace parm

@@ -74,17 +74,39 @@ __device__ __forceinline__ double sqnorm(const Vec<D> &r) {
 }
 
 // Minimum image of one component, equal to IEEE remainder(dx, L) bit for bit
-// (OriginBox::diff, src/box.hpp:68-78,103).  n = rint(dx * (1/L)) is a guess
-// at the nearest integer to dx/L.  r = fma(-n, L, dx) is dx - n L rounded
-// once.  Rounding is monotone and L/2 is representable, so |r| < L/2 implies
-// the exact dx - n L is below L/2 too: n is then the nearest integer, and the
-// exact result (remainder's, always representable) is what fma returned.
-// Otherwise (|r| >= L/2: a wrong guess or a tie) the library remainder runs.
+// (OriginBox::diff, src/box.hpp:68-78,103).
+//
+// remainder(dx, L) = dx - n* L with n* the integer nearest dx/L (ties to
+// even), and that value is always representable, so fma(-n*, L, dx) returns
+// it exactly.  n = rint(dx * (1/L)) is within one of n* whenever |dx/L| <
+// 2^50.  For any other integer m, |dx - m L| >= L/2 exactly, and rounding is
+// monotone with L/2 representable, so |fma(-m, L, dx)| >= L/2 too.  Hence:
+// if |fma(-n, L, dx)| < L/2, n = n*.  Otherwise n* is the one of n-1, n,
+// n+1 whose result is below L/2, or, on an exact tie (two results of
+// magnitude L/2), the even one.
 __device__ __forceinline__ double min_image1(double dx, double L, double invL,
                                              double halfL) {
-    double n = rint(dx * invL);
+    const double n = rint(dx * invL);
     double r = fma(-n, L, dx);
-    if (__builtin_expect(!(fabs(r) < halfL), 0)) r = remainder(dx, L);
+    if (__builtin_expect(!(fabs(r) < halfL), 0)) {
+        double best = r, bestn = n;
+        bool found = false;
+        for (int d = -1; d <= 1; d++) {
+            const double m = n + (double)d;
+            const double rm = fma(-m, L, dx);
+            const double am = fabs(rm);
+            if (am < halfL) {
+                best = rm;
+                bestn = m;
+                found = true;
+            } else if (!found && am == halfL && rint(m * 0.5) == m * 0.5) {
+                best = rm;
+                bestn = m;
+            }
+        }
+        (void)bestn;
+        r = best;
+    }
     return r;
 }
 
@@ -98,6 +120,24 @@ __device__ __forceinline__ Vec<D> diff(const Vec<D> &a, const Vec<D> &b,
                             box.halfL[k]);
     return r;
 }
+
+// NeighborList pair predicate (src/trackers.cpp:63-67), bit for bit:
+//   r = diff(x_i, x_j), d2 = (x^2 + y^2) + z^2,
+//   in list  <=>  sqrt(d2) < (d_i + d_j)/2 + skin   (correctly rounded sqrt).
+// sqrt is only evaluated when d2 is within 1e-14 (relative) of c^2; outside
+// that band the comparison with c^2 decides it with margin to spare.
+template <int D>
+__device__ __forceinline__ bool in_list(const Vec<D> &xi, const Vec<D> &xj, double di,
+                                        double dj, const Box &box, double skin) {
+    const Vec<D> r = diff<D>(xi, xj, box);
+    const double d2 = sqnorm<D>(r);
+    const double c = (di + dj) / 2 + skin;
+    const double c2 = c * c;
+    if (d2 < c2 * (1.0 - 1e-14)) return true;
+    if (d2 > c2 * (1.0 + 1e-14)) return false;
+    return sqrt(d2) < c;
+}
+
 
 // 1/x to ~1 ulp: hardware estimate and two Newton-Raphson steps.
 __device__ __forceinline__ double rcp64(double x) {

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include <cstdint>
 
 #include "common.hpp"
@@ -32,10 +34,10 @@ hipError_t launch_cell_bounds(int n, int ncell, const uint32_t *keys, int32_t *s
 // Builds the full list (and xref = pos); stats[0] = sum of counts, stats[1] =
 // max count; part holds 2 * build_blocks(n) scratch words.
 int build_blocks(int n);
-hipError_t launch_build_list(int ndim, int n, const void *pos, const int32_t *rank,
-                             const double *diam, const int32_t *type,
+hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
+                             const int32_t *rank, const double *diam, const int32_t *type,
                              const uint32_t *keys, const int32_t *cstart,
-                             const CellGrid &g, const Box &box, double skin,
+                             const CellGrid &g, const Box &box, double skin, double dmax,
                              int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                              int32_t *ncount, void *xref, unsigned long long *part,
                              unsigned long long *stats, uint32_t *flag,
@@ -109,7 +111,8 @@ struct StepCoef {
 
 // Reduction scratch: one partial per workgroup, reduced by k_finalize.
 struct ReduceBuf {
-    double *part;                 // nblocks * kRedWidth doubles
+    double *part;                 // kRedWidth fields x pstride workgroups
+    int pstride;                  // field stride (>= workgroups of any launch)
     double *out;                  // kRedWidth doubles (final sums)
     uint32_t *flag;               // flag word
     double skin;                  // skin test threshold
@@ -137,5 +140,52 @@ hipError_t launch_download(int ndim, int what, const AtomArrays &A,
                            const int32_t *orig, double *dst, hipStream_t s);
 int reduce_blocks(int n);
 int force_blocks(int n, int tpa);  // workgroups of a force launch (also partials)
+
+// ---------------------------------------------------------------- batch.hip
+// Geometry of one system of a batch.
+struct BatchGeom {
+    Box box;
+    CellGrid grid;
+    int off;       // first atom of the system in the concatenated arrays
+    int n;         // atoms
+    int cell_off;  // first cell in the concatenated cell-start array
+    int pad;
+};
+
+struct BatchArrays {
+    int ntot;              // atoms over all systems
+    const void *pos_in;    // PosStore<D>::T*
+    void *pos_out;
+    double *v[3], *a[3], *f[3];
+    const double *m;
+    const int32_t *type;   // pair-table type
+    const int32_t *sys;    // system of each atom
+    const double *diam;    // list diameter of each atom
+    const void *xref;      // positions at each system's last rebuild
+    void *xref_w;          // the same buffer, written by the rebuild
+    double *disp;          // |x - xref| per atom (skin test input)
+    const BatchGeom *geom; // per system
+};
+
+hipError_t launch_batch_force(int ndim, int precision, int stage, bool energy,
+                              const BatchArrays &A, const ListView &Lv, const PairView &P,
+                              const StepCoef &c, double *eatom, hipStream_t s);
+hipError_t launch_batch_drift(int ndim, bool damped, const BatchArrays &A,
+                              const StepCoef &c, hipStream_t s);
+hipError_t launch_batch_disp(int ndim, const BatchArrays &A, hipStream_t s);
+hipError_t launch_batch_skin(int nsys, const BatchArrays &A, double skin, uint32_t *flags,
+                             uint32_t *gflag, hipStream_t s);
+hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const void *pos,
+                                int force, int count_rebuild, double skin,
+                                int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
+                                int32_t *ncount, int32_t *keys, int32_t *cellidx,
+                                int32_t *cstart, uint32_t *flags, uint32_t *which,
+                                unsigned long long *npairs, uint32_t *gflag, hipStream_t s);
+hipError_t launch_batch_sys_sums(int nsys, const BatchArrays &A, const double *eatom,
+                                 double *U, double *contacts, hipStream_t s);
+constexpr int kBatchMaxCellsPerSystem = 4096;
+
+// Sets the calling thread's parm_last_error() message; returns code.
+int set_error(int code, const std::string &msg);
 
 }  // namespace parm

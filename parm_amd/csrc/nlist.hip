@@ -194,24 +194,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_bounds(
 //   in list  <=>  sqrt(d2) < (d_i + d_j)/2 + skin   (correctly rounded sqrt).
 // sqrt is only evaluated when d2 is within 1e-14 (relative) of c^2; outside
 // that band the comparison with c^2 decides it with margin to spare.
-template <int D>
-__device__ __forceinline__ bool in_list(const Vec<D> &xi, const Vec<D> &xj, double di,
-                                        double dj, const Box &box, double skin) {
-    const Vec<D> r = diff<D>(xi, xj, box);
-    const double d2 = sqnorm<D>(r);
-    const double c = (di + dj) / 2 + skin;
-    const double c2 = c * c;
-    if (d2 < c2 * (1.0 - 1e-14)) return true;
-    if (d2 > c2 * (1.0 + 1e-14)) return false;
-    return sqrt(d2) < c;
-}
-
-template <int D, int G>
+template <int D, int G, int U>
 __global__ __launch_bounds__(kBlock) void k_build_list(
     int n, const typename PosStore<D>::T *__restrict__ pos,
+    const typename PosStore<D>::T *__restrict__ xw,
     const int32_t *__restrict__ rank, const double *__restrict__ diam,
     const int32_t *__restrict__ type, const uint32_t *__restrict__ keys,
-    const int32_t *__restrict__ cstart, CellGrid g, Box box, double skin,
+    const int32_t *__restrict__ cstart, CellGrid g, Box box, double skin, double dmax,
     int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
     int32_t *__restrict__ ncount, typename PosStore<D>::T *__restrict__ xref,
     unsigned long long *__restrict__ part /* per workgroup: sum, max */,
@@ -231,6 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
             if (sub == 0) ncount[i] = 0;
         } else {
             const Vec<D> xi = PosStore<D>::load(pi);
+            const Vec<D> wi = PosStore<D>::load(xw[i]);
             const double di = diam[i];
             uint32_t key = keys[i];
             int ci[3] = {0, 0, 0};
@@ -240,47 +230,115 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
                 key /= (uint32_t)g.nc[k];
             }
             const bool wide0 = g.nc[0] >= 3;
-            const int oyl = (D >= 2 && g.nc[1] >= 3) ? -1 : 0;
-            const int ozl = (D == 3 && g.nc[2] >= 3) ? -1 : 0;
-            int cnt = 0;
-            for (int oz = ozl; oz <= -ozl; oz++) {
-                for (int oy = oyl; oy <= -oyl; oy++) {
-                    int q1 = ci[1] + oy, q2 = ci[2] + oz;
-                    if (D >= 2) q1 = q1 < 0 ? q1 + g.nc[1] : (q1 >= g.nc[1] ? q1 - g.nc[1] : q1);
-                    if (D == 3) q2 = q2 < 0 ? q2 + g.nc[2] : (q2 >= g.nc[2] ? q2 - g.nc[2] : q2);
-                    const int rowc = (D == 3 ? q2 * g.nc[1] + q1 : q1) * g.nc[0];
-                    // the row's x cells as at most two contiguous cell ranges
-                    int c0[2], c1[2], nr = 1;
-                    if (!wide0) {
-                        c0[0] = 0;
-                        c1[0] = 0;
-                    } else if (ci[0] == 0) {
-                        c0[0] = 0;
-                        c1[0] = 1;
-                        c0[1] = g.nc[0] - 1;
-                        c1[1] = g.nc[0] - 1;
-                        nr = 2;
-                    } else if (ci[0] == g.nc[0] - 1) {
-                        c0[0] = ci[0] - 1;
-                        c1[0] = ci[0];
-                        c0[1] = 0;
-                        c1[1] = 0;
-                        nr = 2;
-                    } else {
-                        c0[0] = ci[0] - 1;
-                        c1[0] = ci[0] + 1;
+            const int ny = (D >= 2 && g.nc[1] >= 3) ? 3 : 1;
+            const int nz = (D == 3 && g.nc[2] >= 3) ? 3 : 1;
+            const int nrows = ny * nz;
+            // The x cells of each row as at most two contiguous cell ranges
+            // (the second one across the periodic boundary, shifted by -L or
+            // +L).  Lane `sub` of the group loads the slot bounds of rows
+            // sub and sub + G, so the group's loads are in flight together;
+            // the loop below reads them back with shuffles.
+            int xa0 = 0, xb0 = 0, xa1 = 0, xb1 = -1;
+            double shx1 = 0.0;
+            if (wide0) {
+                if (ci[0] == 0) {
+                    xa0 = 0; xb0 = 1; xa1 = g.nc[0] - 1; xb1 = g.nc[0] - 1;
+                    shx1 = -box.L[0];
+                } else if (ci[0] == g.nc[0] - 1) {
+                    xa0 = ci[0] - 1; xb0 = ci[0]; xa1 = 0; xb1 = 0;
+                    shx1 = box.L[0];
+                } else {
+                    xa0 = ci[0] - 1; xb0 = ci[0] + 1;
+                }
+            }
+            auto row_base = [&](int r, double *sh) {
+                const int oy = (ny == 3) ? (r % 3) - 1 : 0;
+                const int oz = (nz == 3) ? (r / ny) - 1 : 0;
+                int q1 = ci[1] + oy, q2 = ci[2] + oz;
+                sh[1] = 0.0;
+                sh[2] = 0.0;
+                if (D >= 2) {
+                    if (q1 < 0) { q1 += g.nc[1]; sh[1] = -box.L[1]; }
+                    else if (q1 >= g.nc[1]) { q1 -= g.nc[1]; sh[1] = box.L[1]; }
+                }
+                if (D == 3) {
+                    if (q2 < 0) { q2 += g.nc[2]; sh[2] = -box.L[2]; }
+                    else if (q2 >= g.nc[2]) { q2 -= g.nc[2]; sh[2] = box.L[2]; }
+                }
+                return (D == 3 ? q2 * g.nc[1] + q1 : q1) * g.nc[0];
+            };
+            int bnd[2][4];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int r = sub + t * G;
+                bnd[t][0] = bnd[t][1] = bnd[t][2] = bnd[t][3] = 0;
+                if (r < nrows) {
+                    double shd[3];
+                    const int rowc = row_base(r, shd);
+                    bnd[t][0] = cstart[rowc + xa0];
+                    bnd[t][1] = cstart[rowc + xb0 + 1];
+                    if (xb1 >= 0) {
+                        bnd[t][2] = cstart[rowc + xa1];
+                        bnd[t][3] = cstart[rowc + xb1 + 1];
                     }
-                    for (int rr = 0; rr < nr; rr++) {
-                        const int a = cstart[rowc + c0[rr]];
-                        const int b = cstart[rowc + c1[rr] + 1];
-                        for (int base = a; base < b; base += G) {
-                            const int j = base + sub;
+                }
+            }
+            int cnt = 0;
+            for (int r = 0; r < nrows; r++) {
+                double sh[3];
+                (void)row_base(r, sh);
+                const int t = r / G, owner = gbase + (r % G);
+                int rb[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    rb[k] = __shfl(t == 0 ? bnd[0][k] : bnd[1][k], owner);
+                for (int rr = 0; rr < 2; rr++) {
+                    const int a = rb[2 * rr], b = rb[2 * rr + 1];
+                    sh[0] = rr == 0 ? 0.0 : shx1;
+                    // U candidates per lane per round, their loads issued
+                    // together; appended in candidate order
+                    for (int base = a; base < b; base += G * U) {
+                        typename PosStore<D>::T pw[U];
+                        double dj[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int j = base + u * G + sub;
+                            const int jj = j < b ? j : i;
+                            pw[u] = xw[jj];
+                            dj[u] = diam[jj];
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int j = base + u * G + sub;
                             bool pass = false;
-                            if (j < b && j != i)
-                                pass = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
-                                                  diam[j], box, skin);
+                            if (j < b && j != i) {
+                                // |r|^2 from the wrapped copies and the row's
+                                // periodic shift: equal to the exact d2 up
+                                // to the rounding of the copies, far below
+                                // the 1e-6 band; inside the band the
+                                // reference's predicate decides on the
+                                // unwrapped positions.
+                                const Vec<D> wj = PosStore<D>::load(pw[u]);
+                                double d2w = 0.0;
+#pragma unroll
+                                for (int k = 0; k < D; k++) {
+                                    double dk = wi.c[k] - (wj.c[k] + sh[k]);
+                                    // a collapsed dimension (one cell) keeps
+                                    // every image: take the nearest one
+                                    if (g.nc[k] < 3) dk -= box.L[k] * rint(dk * box.invL[k]);
+                                    d2w = fma(dk, dk, d2w);
+                                }
+                                const double c = (di + dj[u]) / 2 + skin;
+                                const double c2 = c * c;
+                                if (d2w < c2 * (1.0 - 1e-6))
+                                    pass = true;
+                                else if (d2w <= c2 * (1.0 + 1e-6))
+                                    pass = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
+                                                      dj[u], box, skin);
+                            }
                             const unsigned long long m =
-                                (__ballot(pass) >> gbase) & ((G == 64) ? ~0ull : ((1ull << G) - 1ull));
+                                (__ballot(pass) >> gbase) &
+                                ((G == 64) ? ~0ull : ((1ull << G) - 1ull));
                             if (pass) {
                                 const int q = cnt + __popcll(m & below);
                                 if (q < cap)
@@ -321,6 +379,25 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
         part[2 * (size_t)blockIdx.x] = s2;
         part[2 * (size_t)blockIdx.x + 1] = m2;
     }
+}
+
+// Wrapped copy of the positions, frac(x / L) * L per component, computed
+// with the same floor as the cell keys: the list build's prefilter uses it.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_wrap(int n,
+                                                 const typename PosStore<D>::T *__restrict__ pos,
+                                                 CellGrid g, Box box,
+                                                 typename PosStore<D>::T *__restrict__ xw) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Vec<D> x = PosStore<D>::load(pos[i]);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        double s = x.c[k] * g.invL[k];
+        s -= floor(s);
+        x.c[k] = s * box.L[k];
+    }
+    xw[i] = PosStore<D>::make(x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_build_stats(
@@ -432,30 +509,36 @@ hipError_t launch_cell_bounds(int n, int ncell, const uint32_t *keys, int32_t *s
     return hipGetLastError();
 }
 
-constexpr int kBuildLanes = 8;  // lanes per slot in k_build_list
+constexpr int kBuildLanes = 8;   // lanes per slot in k_build_list
+constexpr int kBuildUnroll = 4;  // candidates per lane per round
 
 int build_blocks(int n) { return (int)(((long)n * kBuildLanes + kBlock - 1) / kBlock); }
 
-hipError_t launch_build_list(int ndim, int n, const void *pos, const int32_t *rank,
-                             const double *diam, const int32_t *type,
+hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
+                             const int32_t *rank, const double *diam, const int32_t *type,
                              const uint32_t *keys, const int32_t *cstart,
-                             const CellGrid &g, const Box &box, double skin,
+                             const CellGrid &g, const Box &box, double skin, double dmax,
                              int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                              int32_t *ncount, void *xref, unsigned long long *part,
                              unsigned long long *stats, uint32_t *flag,
                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int nb = build_blocks(n);
-    if (ndim == 3)
-        hipLaunchKernelGGL((k_build_list<3, kBuildLanes>), dim3(nb), dim3(kBlock), 0, s,
-                           n, (const double4 *)pos, rank, diam, type, keys, cstart, g,
-                           box, skin, type_bits, tpa, cap, npad, nbr, ncount,
-                           (double4 *)xref, part, flag);
-    else
-        hipLaunchKernelGGL((k_build_list<2, kBuildLanes>), dim3(nb), dim3(kBlock), 0, s,
-                           n, (const double2 *)pos, rank, diam, type, keys, cstart, g,
-                           box, skin, type_bits, tpa, cap, npad, nbr, ncount,
-                           (double2 *)xref, part, flag);
+    if (ndim == 3) {
+        hipLaunchKernelGGL(k_wrap<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
+                           (const double4 *)pos, g, box, (double4 *)xw);
+        hipLaunchKernelGGL((k_build_list<3, kBuildLanes, kBuildUnroll>), dim3(nb), dim3(kBlock), 0, s,
+                           n, (const double4 *)pos, (const double4 *)xw, rank, diam, type,
+                           keys, cstart, g, box, skin, dmax, type_bits, tpa, cap, npad,
+                           nbr, ncount, (double4 *)xref, part, flag);
+    } else {
+        hipLaunchKernelGGL(k_wrap<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
+                           (const double2 *)pos, g, box, (double2 *)xw);
+        hipLaunchKernelGGL((k_build_list<2, kBuildLanes, kBuildUnroll>), dim3(nb), dim3(kBlock), 0, s,
+                           n, (const double2 *)pos, (const double2 *)xw, rank, diam, type,
+                           keys, cstart, g, box, skin, dmax, type_bits, tpa, cap, npad,
+                           nbr, ncount, (double2 *)xref, part, flag);
+    }
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, nb, part, stats);
     return hipGetLastError();
 }

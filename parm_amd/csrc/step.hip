@@ -19,6 +19,7 @@
 // scatter.  Energies, virials and contact counts are halved.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "pairs.hpp"
 
 namespace parm {
 
@@ -57,11 +58,13 @@ __device__ __forceinline__ void block_partial(double (&v)[kRedWidth], Top2 t,
     }
     __syncthreads();
     const int k = threadIdx.x;
-    double *dst = R.part + (size_t)blockIdx.x * kRedWidth;
+    // field-major partials: field k of workgroup b at part[k * pstride + b]
+    double *dst = R.part + blockIdx.x;
+    const size_t ps = (size_t)R.pstride;
     if (SUMS && k < kTop1) {
         double p = 0.0;
         for (int q = 0; q < kBlock / kWave; q++) p += sh[q][k];
-        dst[k] = p;
+        dst[k * ps] = p;
     }
     if (SKIN && k == kTop1) {
         Top2 bt;
@@ -73,26 +76,32 @@ __device__ __forceinline__ void block_partial(double (&v)[kRedWidth], Top2 t,
             bt.merge(o);
             nf += sh[q][kNonFinite];
         }
-        dst[kTop1] = bt.a;
-        dst[kTop2] = bt.b;
-        dst[kNonFinite] = nf;
+        dst[kTop1 * ps] = bt.a;
+        dst[kTop2 * ps] = bt.b;
+        dst[kNonFinite * ps] = nf;
     }
 }
 
-// One workgroup: thread t reduces partials t, t + kBlock, ... (Neumaier),
-// then a fixed butterfly and a fixed 4-wave combine (compensated two-sum).
+// One workgroup of kFinal threads: thread t reduces partials t, t + kFinal,
+// ... of each field (Neumaier; the loads of a field are coalesced across the
+// workgroup), then a fixed butterfly per wave and a fixed combine over waves
+// (compensated two-sum), so the result is deterministic.
+constexpr int kFinal = 1024;
+
 template <bool SUMS, bool SKIN>
-__global__ __launch_bounds__(kBlock) void k_finalize(int nparts, ReduceBuf R) {
+__global__ __launch_bounds__(kFinal) void k_finalize(int nparts, ReduceBuf R) {
     const int tid = threadIdx.x;
     const int lane = tid % kWave, w = tid / kWave;
+    constexpr int NW = kFinal / kWave;
+    const size_t ps = (size_t)R.pstride;
     if constexpr (SUMS) {
         CSum cs[kTop1];
-        for (int b = tid; b < nparts; b += kBlock) {
-            const double *src = R.part + (size_t)b * kRedWidth;
+#pragma unroll 4
+        for (int b = tid; b < nparts; b += kFinal) {
 #pragma unroll
             for (int k = 0; k < kTop1; k++) {
                 KSum q{cs[k].s, cs[k].c};
-                q.add(src[k]);
+                q.add(R.part[k * ps + b]);
                 cs[k] = CSum{q.s, q.c};
             }
         }
@@ -104,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int nparts, ReduceBuf R) {
                 cs[k] = csum_merge(cs[k], o);
             }
         }
-        __shared__ double fin[kBlock / kWave][2 * kTop1];
+        __shared__ double fin[NW][2 * kTop1];
         if (lane == 0) {
 #pragma unroll
             for (int k = 0; k < kTop1; k++) {
@@ -115,39 +124,35 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int nparts, ReduceBuf R) {
         __syncthreads();
         if (tid < kTop1) {
             CSum tot;
-            for (int q = 0; q < kBlock / kWave; q++)
+            for (int q = 0; q < NW; q++)
                 tot = csum_merge(tot, CSum{fin[q][2 * tid], fin[q][2 * tid + 1]});
             R.out[tid] = tot.s + tot.c;
         }
     }
     if constexpr (SKIN) {
         Top2 bt;
-        CSum nf;
-        for (int b = tid; b < nparts; b += kBlock) {
-            const double *src = R.part + (size_t)b * kRedWidth;
+        double nf = 0.0;
+#pragma unroll 4
+        for (int b = tid; b < nparts; b += kFinal) {
             Top2 o;
-            o.a = src[kTop1];
-            o.b = src[kTop2];
+            o.a = R.part[kTop1 * ps + b];
+            o.b = R.part[kTop2 * ps + b];
             bt.merge(o);
-            KSum q{nf.s, nf.c};
-            q.add(src[kNonFinite]);
-            nf = CSum{q.s, q.c};
+            nf += R.part[kNonFinite * ps + b];
         }
         bt = wave_top2(bt);
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1)
-            nf = csum_merge(nf, CSum{__shfl_xor(nf.s, off), __shfl_xor(nf.c, off)});
-        __shared__ double fin2[kBlock / kWave][3];
+        nf = wave_sum(nf);
+        __shared__ double fin2[NW][3];
         if (lane == 0) {
             fin2[w][0] = bt.a;
             fin2[w][1] = bt.b;
-            fin2[w][2] = nf.s + nf.c;
+            fin2[w][2] = nf;
         }
         __syncthreads();
         if (tid == 0) {
             Top2 tt;
             double nft = 0.0;
-            for (int q = 0; q < kBlock / kWave; q++) {
+            for (int q = 0; q < NW; q++) {
                 Top2 o;
                 o.a = fin2[q][0];
                 o.b = fin2[q][1];
@@ -171,179 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int nparts, ReduceBuf R) {
 
 template <bool SUMS, bool SKIN>
 static void launch_finalize(int nparts, const ReduceBuf &R, hipStream_t s) {
-    hipLaunchKernelGGL((k_finalize<SUMS, SKIN>), dim3(1), dim3(kBlock), 0, s, nparts, R);
-}
-
-// ------------------------------------------------------------ pair math
-struct PairAcc {
-    double e = 0.0, w = 0.0, s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double contacts = 0.0, overlaps = 0.0;
-};
-
-// FP64 LJAttractRepulsePair on one visit; returns false if outside cutoff.
-// Cutoff: the reference skips when d2 / sig^2 > cut^2; lim_lo / lim_hi bracket
-// that boundary so the division only runs within 1e-14 of it.
-template <int D, bool ENERGY>
-__device__ __forceinline__ void lj64(const PairParam &pp, const Vec<D> &r,
-                                     double (&f)[D], PairAcc &acc) {
-    const double d2 = sqnorm<D>(r);
-    bool in = d2 < pp.lim_lo ||
-              (!(d2 > pp.lim_hi) && !(d2 / pp.sig2 > pp.cut2));
-    if (!in) return;
-    const double idsq = rcp64(d2);
-    const double ir = pp.sig2 * idsq;         // sigma^2 / r^2
-    const double rsix = ir * ir * ir;         // sigma^6 / r^6
-    const double fmagR = pp.p6 * rsix * (rsix - 1.0);  // 12 eps s (s - 1)
-    const double fs = fmagR * idsq;
-#pragma unroll
-    for (int k = 0; k < D; k++) f[k] = fma(fs, r.c[k], f[k]);
-    if (ENERGY) {
-        const double mid = 1.0 - rsix;
-        const double e = pp.eps * (mid * mid) - pp.p5;
-        acc.e += e;
-        acc.w += fmagR;
-#pragma unroll
-        for (int a = 0; a < D; a++)
-#pragma unroll
-            for (int b = 0; b < D; b++) acc.s[a * D + b] += r.c[a] * (r.c[b] * fs);
-        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
-        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
-    }
-}
-
-// FP64 RepulsionPair: same operations as the reference, so each pair force
-// is bit-identical to the reference's for n == 2 (pow(x,1) = x, pow(x,2) = x*x).
-template <int D, bool ENERGY, bool HARM>
-__device__ __forceinline__ void rep64(const PairParam &pp, const Vec<D> &r,
-                                      double (&f)[D], PairAcc &acc) {
-    const double d2 = sqnorm<D>(r);
-    if (d2 > pp.sig2) return;
-    const double R = sqrt(d2);
-    const double x = 1.0 - (R / pp.sig);
-    const double px = HARM ? x : pow(x, pp.p6);
-    const double fs = pp.eps * px / pp.sig / R;
-#pragma unroll
-    for (int k = 0; k < D; k++) f[k] = fma(r.c[k], fs, f[k]);
-    if (ENERGY) {
-        const double e = pp.eps * (HARM ? x * x : pow(x, pp.p5)) / pp.p5;
-        acc.e += e;
-        double rf = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; k++) rf += r.c[k] * (r.c[k] * fs);
-        acc.w += rf;
-#pragma unroll
-        for (int a = 0; a < D; a++)
-#pragma unroll
-            for (int b = 0; b < D; b++) acc.s[a * D + b] += r.c[a] * (r.c[b] * fs);
-        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
-        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
-    }
-}
-
-// Mixed precision: the displacement is formed in FP64 (exact minimum image),
-// then r^2, r^-6 and the force are evaluated in FP32.
-template <int D, bool ENERGY>
-__device__ __forceinline__ void lj32(const PairParam &pp, const Vec<D> &r,
-                                     float (&f)[D], PairAcc &acc) {
-    float rf[D];
-#pragma unroll
-    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
-    float d2 = rf[0] * rf[0];
-#pragma unroll
-    for (int k = 1; k < D; k++) d2 = fmaf(rf[k], rf[k], d2);
-    // The LJ force is not shifted to zero at the cutoff, so the in/out
-    // decision must not flip with FP32 rounding: within 1e-5 of the cutoff it
-    // is taken in FP64 exactly as the FP64 path takes it.
-    if (d2 > pp.lim_f * 0.99999f) {
-        if (d2 > pp.lim_f * 1.00001f) return;
-        const double d2d = sqnorm<D>(r);
-        if (!(d2d < pp.lim_lo || (!(d2d > pp.lim_hi) && !(d2d / pp.sig2 > pp.cut2))))
-            return;
-    }
-    const float idsq = __builtin_amdgcn_rcpf(d2);
-    const float ir = pp.sig2_f * idsq;
-    const float rsix = ir * ir * ir;
-    const float fmagR = pp.p6_f * rsix * (rsix - 1.0f);
-    const float fs = fmagR * idsq;
-#pragma unroll
-    for (int k = 0; k < D; k++) f[k] = fmaf(fs, rf[k], f[k]);
-    if (ENERGY) {
-        const float mid = 1.0f - rsix;
-        const double e = (double)(pp.eps_f * (mid * mid) - pp.p5_f);
-        acc.e += e;
-        acc.w += (double)fmagR;
-#pragma unroll
-        for (int a = 0; a < D; a++)
-#pragma unroll
-            for (int b = 0; b < D; b++)
-                acc.s[a * D + b] += (double)(rf[a] * (rf[b] * fs));
-        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
-        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
-    }
-}
-
-template <int D, bool ENERGY, bool HARM>
-__device__ __forceinline__ void rep32(const PairParam &pp, const Vec<D> &r,
-                                      float (&f)[D], PairAcc &acc) {
-    float rf[D];
-#pragma unroll
-    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
-    float d2 = rf[0] * rf[0];
-#pragma unroll
-    for (int k = 1; k < D; k++) d2 = fmaf(rf[k], rf[k], d2);
-    if (d2 > pp.sig2_f * 0.99999f) {  // contact decided in FP64 near sigma
-        if (d2 > pp.sig2_f * 1.00001f) return;
-        if (sqnorm<D>(r) > pp.sig2) return;
-    }
-    const float R = __builtin_sqrtf(d2);
-    const float x = fmaxf(1.0f - R * pp.isig_f, 0.0f);
-    const float px = HARM ? x : powf(x, pp.p6_f);
-    const float fs = pp.eps_f * px * pp.isig_f * __builtin_amdgcn_rcpf(R);
-#pragma unroll
-    for (int k = 0; k < D; k++) f[k] = fmaf(rf[k], fs, f[k]);
-    if (ENERGY) {
-        const double e =
-            (double)(pp.eps_f * (HARM ? x * x : powf(x, pp.p5_f)) / pp.p5_f);
-        acc.e += e;
-        float rfs = 0.0f;
-#pragma unroll
-        for (int k = 0; k < D; k++) rfs = fmaf(rf[k], rf[k] * fs, rfs);
-        acc.w += (double)rfs;
-#pragma unroll
-        for (int a = 0; a < D; a++)
-#pragma unroll
-            for (int b = 0; b < D; b++)
-                acc.s[a * D + b] += (double)(rf[a] * (rf[b] * fs));
-        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
-        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
-    }
-}
-
-// ------------------------------------------------------------ step kernels
-__device__ __forceinline__ bool is_fixed(double m) { return m <= 0 || isinf(m); }
-
-// First half of a step for one atom: the reference's first loop.
-template <int D, bool DAMPED>
-__device__ __forceinline__ void drift_atom(Vec<D> &x, double (&v)[D],
-                                           const double (&a)[D], bool fixed,
-                                           const StepCoef &c) {
-    if (fixed) {
-#pragma unroll
-        for (int k = 0; k < D; k++) v[k] = 0.0;
-        return;
-    }
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        if (!DAMPED) {
-            // x += v*dt + a*(dt*dt/2) ; v += a*(dt/2)
-            x.c[k] = x.c[k] + (v[k] * c.k0 + a[k] * c.k1);
-            v[k] = v[k] + a[k] * c.k2;
-        } else {
-            // x += v*(c1 dt) + a*(c2 dt dt) ; v = v*c0 + a*(dt (c1 - c2))
-            x.c[k] = x.c[k] + (v[k] * c.k0 + a[k] * c.k1);
-            v[k] = v[k] * c.k3 + a[k] * c.k2;
-        }
-    }
+    hipLaunchKernelGGL((k_finalize<SUMS, SKIN>), dim3(1), dim3(kFinal), 0, s, nparts, R);
 }
 
 template <int D>
@@ -357,24 +190,6 @@ __device__ __forceinline__ void skin_atom(const Vec<D> &x, const void *xref, int
     const double dist = sqrt(sqnorm<D>(d));
     t.add(dist);
     if (!isfinite(dist)) nonfinite += 1.0;
-}
-
-// One pair visit: dispatch on kind and precision.
-template <int D, int KIND, int PREC, bool ENERGY, bool HARM>
-__device__ __forceinline__ void visit(const PairParam &pp, const Vec<D> &r,
-                                      double (&f64)[D], float (&f32)[D],
-                                      PairAcc &acc) {
-    if (PREC == kFP64) {
-        if (KIND == kPairLJ)
-            lj64<D, ENERGY>(pp, r, f64, acc);
-        else
-            rep64<D, ENERGY, HARM>(pp, r, f64, acc);
-    } else {
-        if (KIND == kPairLJ)
-            lj32<D, ENERGY>(pp, r, f32, acc);
-        else
-            rep32<D, ENERGY, HARM>(pp, r, f32, acc);
-    }
 }
 
 // TPA lanes share one atom: lane `sub` visits list entries sub, sub + TPA, ...

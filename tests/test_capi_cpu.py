@@ -45,3 +45,12 @@ def test_no_gpu_means_loud_failure():
 def test_invalid_arguments_rejected_before_device():
     with pytest.raises(ValueError):
         capi.System(4, [1.0])
+
+
+def test_batch_no_gpu_means_loud_failure():
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is visible here")
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        capi.Batch(3, [4, 8], [[2.0] * 3, [3.0] * 3], 0.2)
+    with pytest.raises(ValueError):
+        capi.Batch(3, [4, 0], [[2.0] * 3, [3.0] * 3], 0.2)
