@@ -20,6 +20,13 @@ N * (192 + 4 * nbar) with nbar the mean full-list length (FP64 state read and
 written once, one int32 list index per entry, neighbour gathers counted as
 cache hits).  Peak: 8 TB/s HBM3E.
 
+Batched packings (reported under "batched"): 4096 cubic systems with atom
+counts uniform in [64, 4096], bidisperse 1.0 / 1.4 harmonic spheres at
+packing fraction 0.64 from uniform random positions, damped Verlet dt = 0.01
+(damping 1.0), 2000 steps, mixed precision, all systems in one batch with
+graph-replayed steps; beside it the same systems run one after another
+through the single-system path (a sample of them).
+
 CPU baseline: the oracle's restatement of ParM's single-core CPU path
 (AoS atoms, half list, O(N^2) rebuild), timed on this host on a bounded
 sample (see the "sample" field).
@@ -131,6 +138,93 @@ def cpu_baseline(x, v, L, skin, dt, gpu_steps_per_rebuild):
     }
 
 
+def packings(nsys, nmin, nmax, phi, seed):
+    """Batched workload: nsys cubic boxes, atom counts uniform in [nmin, nmax],
+    bidisperse diameters 1.0 / 1.4 (50/50), packing fraction phi, uniform
+    random positions (seeded)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, nsys)
+    types, xs, Ls = [], [], []
+    for n in sizes:
+        t = np.zeros(n, np.int32)
+        t[rng.permutation(n)[: n // 2]] = 1
+        sig = np.array([1.0, 1.4])[t]
+        L = float((np.sum(np.pi / 6 * sig ** 3) / phi) ** (1 / 3))
+        types.append(t)
+        xs.append(rng.uniform(0, L, (n, 3)))
+        Ls.append(L)
+    return sizes, np.concatenate(types), np.concatenate(xs), np.asarray(Ls)
+
+
+def batched(args):
+    """Packings: all systems in one batch (graph-replayed steps) vs the same
+    systems run one after another through the single-system path."""
+    prec = capi.MIXED if args.batch_precision == "mixed" else capi.FP64
+    sizes, types, x, Ls = packings(args.batch_systems, 64, 4096, 0.64, args.seed + 1)
+    ntot = int(sizes.sum())
+    log(f"bench: batched packings, {len(sizes)} systems, {ntot} atoms")
+    eps, sig, expo = [1.0, 1.0], [1.0, 1.4], [2.0, 2.0]
+    b = capi.Batch(3, sizes, np.repeat(Ls[:, None], 3, 1), args.batch_skin, prec)
+    b.set_repulsion(eps, sig, expo, types)
+    b.set_state(x=x, v=np.zeros_like(x), a=np.zeros_like(x))
+    b.initialize()
+    U0, _ = b.sums()
+    # autotune (first call of >= 16 steps) and graph capture happen here;
+    # 17 steps = 8 graph replays, leaving the position buffers as captured
+    b.damped(args.batch_dt, args.batch_damping, 17)
+    b.set_state(x=x, v=np.zeros_like(x), a=np.zeros_like(x))
+    b.initialize()
+    w0, _ = b.lists()
+    t0 = time.perf_counter()
+    b.damped(args.batch_dt, args.batch_damping, args.batch_steps)
+    U1, _ = b.sums()  # synchronises
+    wall = time.perf_counter() - t0
+    w1, _ = b.lists()
+    b.close()
+    value = ntot * args.batch_steps / wall
+    log(f"bench: batched {value:.3e} atom-steps/s, {len(sizes) / wall:.1f} packings/s, "
+        f"{1e3 * wall / args.batch_steps:.3f} ms/step")
+    # sequential: the first k systems, one parm_system each, same steps
+    k = min(args.batch_seq_sample, len(sizes))
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    seq_wall, seq_atoms = 0.0, 0
+    for q in range(k):
+        sl = slice(off[q], off[q + 1])
+        n = int(sizes[q])
+        s = capi.System(3, np.ones(n), prec)
+        s.set_box([Ls[q]] * 3)
+        s.nlist_set(args.batch_skin, np.array(sig)[types[sl]])
+        s.set_repulsion(eps, sig, expo, types[sl])
+        s.set_state(x=x[sl], v=np.zeros((n, 3)), a=np.zeros((n, 3)))
+        s.initialize()
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.damped(args.batch_dt, args.batch_damping, args.batch_steps)
+        s.synchronize()
+        seq_wall += time.perf_counter() - t0
+        seq_atoms += n
+        s.close()
+    seq_value = seq_atoms * args.batch_steps / seq_wall
+    log(f"bench: sequential {seq_value:.3e} atom-steps/s over {k} systems")
+    return {
+        "workload": "3D bidisperse harmonic packings (RepulsionPair, exponent 2), damped "
+                    "Verlet from uniform random positions",
+        "systems": int(len(sizes)), "natoms_total": ntot, "natoms_range": [64, 4096],
+        "packing_fraction": 0.64, "dt": args.batch_dt, "damping": args.batch_damping,
+        "skin": args.batch_skin, "steps": args.batch_steps,
+        "precision": args.batch_precision,
+        "value": value, "unit": "atom-timesteps/s",
+        "packings_per_s": len(sizes) / wall, "ms_per_step": 1e3 * wall / args.batch_steps,
+        "rebuilds_per_system_mean": float(np.mean(w1 - w0)),
+        "energy_per_atom_start": float(U0.sum() / ntot),
+        "energy_per_atom_end": float(U1.sum() / ntot),
+        "sequential": {"value": seq_value, "unit": "atom-timesteps/s",
+                       "sample": f"the first {k} systems, one parm_system each, "
+                                 f"{args.batch_steps} steps each, run one after another"},
+        "speedup_vs_sequential": value / seq_value,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +237,14 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--drift-every", type=int, default=100)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--batch-dt", type=float, default=0.01)
+    ap.add_argument("--batch-damping", type=float, default=1.0)
+    ap.add_argument("--batch-skin", type=float, default=0.2)
+    ap.add_argument("--batch-precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--batch-seq-sample", type=int, default=32)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -237,6 +339,9 @@ def main():
         }
         log(f"bench: mixed {value_m:.3e} atom-steps/s ({value_m / value:.2f}x), drift "
             f"{drift_m:.2e}")
+
+    if not args.no_batch:
+        out["batched"] = batched(args)
 
     if not args.no_cpu:
         log("bench: timing the CPU baseline sample")

@@ -8,10 +8,10 @@
 // the device per system, so a step needs no host round trip and a run of
 // steps can be captured in a HIP graph.
 //
-// Per step:  k_batch_rebuild one workgroup per system; exits unless that
+// Per step:  k_batch_force   forces + kick (+ next drift, |x - xref| per atom)
+//            k_batch_rebuild one workgroup per system; exits unless that
 //                            system's skin flag is set: cell sort in LDS,
 //                            exact list build, xref = x
-//            k_batch_force   forces + kick (+ next drift, |x - xref| per atom)
 //            k_batch_skin    one wavefront per system: top-2 of |x - xref|
 //                            -> the system's flag for the next rebuild
 #include "common.hpp"
@@ -207,12 +207,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_skin(int nsys, BatchArrays A,
 // force).  Cell keys and a stable counting sort of the system's atoms by
 // cell in LDS, then the exact list build over the periodic 27-cell
 // stencil (candidates in cell order, then index order: deterministic), then
-// xref = x.
+// xref = x.  With `alt` (the positions one step ahead), |alt - xref| is
+// recomputed for the next skin test.
 constexpr int kBatchMaxCells = kBatchMaxCellsPerSystem;
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
-    BatchArrays A, const typename PosStore<D>::T *__restrict__ pos, int force, int count_rebuild,
+    BatchArrays A, const typename PosStore<D>::T *__restrict__ pos,
+    const typename PosStore<D>::T *__restrict__ alt, int force, int count_rebuild,
     double skin, int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
     int32_t *__restrict__ ncount, int32_t *__restrict__ keys,
     int32_t *__restrict__ cellidx, int32_t *__restrict__ cstart,
@@ -319,6 +321,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
         ncount[i] = cnt < cap ? cnt : cap;
         if (cnt > cap) atomicOr(gflag, kFlagOverflow);
         mycnt += (unsigned long long)cnt;
+        if (alt) {
+            const Vec<D> xa = PosStore<D>::load(alt[i]);
+            Vec<D> dd;
+#pragma unroll
+            for (int d = 0; d < D; d++) dd.c[d] = xa.c[d] - xi.c[d];
+            A.disp[i] = sqrt(sqnorm<D>(dd));
+        }
     }
     unsigned long long sum = mycnt;
 #pragma unroll
@@ -498,7 +507,7 @@ hipError_t launch_batch_skin(int nsys, const BatchArrays &A, double skin, uint32
 }
 
 hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const void *pos,
-                                int force, int count_rebuild, double skin,
+                                const void *alt, int force, int count_rebuild, double skin,
                                 int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                                 int32_t *ncount, int32_t *keys, int32_t *cellidx,
                                 int32_t *cstart, uint32_t *flags, uint32_t *which,
@@ -507,12 +516,12 @@ hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const 
     if (nsys <= 0) return hipSuccess;
     if (ndim == 3)
         hipLaunchKernelGGL(k_batch_rebuild<3>, dim3(nsys), dim3(kBlock), 0, s, A,
-                           (const double4 *)pos, force, count_rebuild,
+                           (const double4 *)pos, (const double4 *)alt, force, count_rebuild,
                            skin, type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx,
                            cstart, flags, which, npairs, gflag);
     else
         hipLaunchKernelGGL(k_batch_rebuild<2>, dim3(nsys), dim3(kBlock), 0, s, A,
-                           (const double2 *)pos, force, count_rebuild,
+                           (const double2 *)pos, (const double2 *)alt, force, count_rebuild,
                            skin, type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx,
                            cstart, flags, which, npairs, gflag);
     return hipGetLastError();

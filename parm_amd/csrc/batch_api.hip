@@ -255,8 +255,11 @@ int read_gflag(parm_batch *b, uint32_t *fl) {
 }
 
 // Rebuild every system (force) or those whose skin flag is set.
-int brebuild(parm_batch *b, bool force, bool count, const void *pos) {
-    PB_HIP(launch_batch_rebuild(b->ndim, b->nsys, b->arrays(), pos, force ? 1 : 0,
+// `alt`: positions one step ahead of `pos`; for a rebuilt system their
+// displacement from the new reference positions is recomputed.
+int brebuild(parm_batch *b, bool force, bool count, const void *pos,
+             const void *alt = nullptr) {
+    PB_HIP(launch_batch_rebuild(b->ndim, b->nsys, b->arrays(), pos, alt, force ? 1 : 0,
                                 count ? 1 : 0, b->skin, b->type_bits, b->tpa, b->cap,
                                 b->npad, b->nbr, b->ncount, b->keys, b->cellidx, b->cstart,
                                 b->flags, b->which, b->npairs, b->gflag, b->stream));
@@ -335,18 +338,20 @@ int bautotune(parm_batch *b) {
     return PARM_OK;
 }
 
-// One fused step of the reference's timestep, shifted by half a step: the
-// rebuild of the systems whose skin test fired at x_q (update_trackers),
-// forces at x_q with the closing kick of step q-1, the drift of step q into
-// the other buffer with |x_{q+1} - xref|, then the skin test at x_{q+1}.
+// One fused step.  The reference's timestep is drift, forces, kick, then
+// update_trackers at the new positions.  Shifted by half a step: forces at
+// x_q with the closing kick of step q-1 and the drift of step q into the other
+// buffer (with |x_{q+1} - xref|), then update_trackers of step q-1: the
+// rebuild at x_q of the systems whose skin test fired at x_q (recomputing
+// |x_{q+1} - xref| for them), then the skin test at x_{q+1}.
 int fused_step(parm_batch *b, bool damped, const StepCoef &c, void *cur, void *nxt) {
-    PB_CHECK(brebuild(b, false, true, cur));
     BatchArrays A = b->arrays();
     A.pos_in = cur;
     A.pos_out = nxt;
     PB_HIP(launch_batch_force(b->ndim, b->prec,
                               damped ? kStageDampedKickDrift : kStageVerletKickDrift, false, A,
                               b->listview(), b->pairview(), c, b->eatom, b->stream));
+    PB_CHECK(brebuild(b, false, true, cur, nxt));
     PB_HIP(launch_batch_skin(b->nsys, b->arrays(), b->skin, b->flags, b->gflag, b->stream));
     return PARM_OK;
 }
@@ -398,11 +403,11 @@ int run_batch(parm_batch *b, bool damped, const StepCoef &c, int64_t nsteps, boo
         PB_CHECK(fused_step(b, damped, c, b->pos, b->pos_alt));
         std::swap(b->pos, b->pos_alt);
     }
-    // last step: list update at x_n, forces, closing kick
-    PB_CHECK(brebuild(b, false, true, b->pos));
+    // last step: forces + closing kick, then update_trackers at x_n
     PB_HIP(launch_batch_force(b->ndim, b->prec, damped ? kStageDampedKick : kStageVerletKick,
                               false, b->arrays(), b->listview(), b->pairview(), c, b->eatom,
                               st));
+    PB_CHECK(brebuild(b, false, true, b->pos));
     uint32_t fl = 0;
     PB_CHECK(read_gflag(b, &fl));
     if (fl & kFlagNonFinite) return bfail(PARM_E_NONFINITE, "a position became non-finite");

@@ -176,7 +176,7 @@ hipError_t launch_batch_disp(int ndim, const BatchArrays &A, hipStream_t s);
 hipError_t launch_batch_skin(int nsys, const BatchArrays &A, double skin, uint32_t *flags,
                              uint32_t *gflag, hipStream_t s);
 hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const void *pos,
-                                int force, int count_rebuild, double skin,
+                                const void *alt, int force, int count_rebuild, double skin,
                                 int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                                 int32_t *ncount, int32_t *keys, int32_t *cellidx,
                                 int32_t *cstart, uint32_t *flags, uint32_t *which,

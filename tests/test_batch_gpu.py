@@ -178,6 +178,32 @@ def test_batch_trajectory_matches_oracle(integrator, ndim):
         assert np.array_equal(f_dev, f_ref) and np.array_equal(l_dev, l_ref), s
 
 
+def test_batch_list_update_order_with_zero_skin():
+    """As the single-system test: with skin 0 every step rebuilds, and the
+    forces at x_{q+1} must use the list tested at x_q (the reference's
+    timestep order), in every system of the batch and through graph replay."""
+    _device_or_fail()
+    print("batch zero skin", flush=True)
+    systems = []
+    for k, n in enumerate([300, 125, 512, 64]):
+        x, L = S.jittered_lattice(n, 0.8, 3, seed=60 + k)
+        systems.append(dict(n=n, L=float(L[0]), x=x,
+                            v=S.gaussian_velocities(n, 3, 1.0, seed=70 + k),
+                            types=np.zeros(n, np.int32), sig=np.ones(n)))
+    b = make_batch(systems, 3, 0.0, lj=True)
+    refs = oracle_systems(systems, 3, 0.0, lj=True)
+    b.initialize()
+    for r in refs:
+        r.initialize()
+    for k in (1, 4, 21):
+        b.verlet(0.005, k)
+        for r in refs:
+            r.verlet(0.005, k)
+        _compare(b, refs, systems, 1e-11, k)
+    w, _ = b.lists()
+    assert np.all(w == 1 + 26)
+
+
 def test_batch_graph_replay_equals_eager(monkeypatch):
     _device_or_fail()
     print("batch graph vs eager", flush=True)

@@ -210,6 +210,39 @@ def test_skin_rebuild_schedule_matches_oracle():
     assert np.array_equal(fd_, fr_) and np.array_equal(ld_, lr_)
 
 
+@pytest.mark.parametrize("integrator", ["verlet", "damped"])
+def test_list_update_order_with_zero_skin(integrator):
+    """Skin 0 rebuilds after every step that moved anything, and the forces
+    of the next step use that list: the reference's order (forces at x_{q+1}
+    with the list tested at x_q, then update_list at x_{q+1}).  Hundreds of
+    pairs cross the cutoff per step here, so a force evaluated with a list
+    rebuilt at x_{q+1} itself would differ far beyond the tolerance."""
+    _device_or_skip()
+    n = 1000
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=77)
+    p = S.lj_params(n)
+    v = S.gaussian_velocities(n, 3, 1.0, seed=78)
+    dev = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.0)
+    dev.set_state(v=v)
+    ref = _oracle_system(O.LJ_ATTRACT_REPULSE, x, L, p, 0.0, np.ones(n))
+    ref.set_state(x=x, v=v, a=np.zeros_like(x))
+    dev.initialize()
+    ref.initialize()
+    for k in (1, 3, 20):
+        if integrator == "verlet":
+            dev.verlet(0.005, k)
+            ref.verlet(0.005, k)
+        else:
+            dev.damped(0.005, 0.5, k)
+            ref.damped(0.005, 0.5, k)
+        xd, vd, _, fd = dev.state()
+        xr, vr, _, fr = ref.state()
+        assert dev.which() == ref.which(), k
+        assert np.abs(xd - xr).max() <= 1e-11 * np.abs(xr).max(), k
+        assert np.abs(fd - fr).max() <= 1e-9 * np.abs(fr).max(), k
+    assert dev.which() == 1 + 24  # the initial build + one per step
+
+
 def test_mixed_precision_force_error():
     _device_or_skip()
     n = 4096
