@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "pairtable.hpp"
+#include "tiles.hpp"
 
 using namespace parm;
 
@@ -77,6 +78,8 @@ void dfree(T *&p) {
 constexpr int kFlagRing = 4;
 // flag words: [0, kFlagRing) step ring, then the list-build overflow flag, a
 // scratch word for reductions that raise no flag, and the on-demand skin test
+// dynamic LDS of one tiled build workgroup, at most (the stencil's atoms)
+constexpr size_t kTiledLdsMax = 56 * 1024;
 constexpr int kSlotBuild = kFlagRing + 1, kSlotScratch = kFlagRing + 2,
               kSlotSkin = kFlagRing + 3;
 
@@ -116,8 +119,16 @@ struct parm_system {
     int cap = 0, npad = 0, type_bits = 0;
     int tpa = 4;            // lanes per atom in the force kernel (autotuned)
     bool tpa_fixed = false, tuned = false;
-    uint32_t *nbr = nullptr;
+    uint32_t *nbr = nullptr;       // untiled: uint32 entries; tiled: uint16 staged slots
     int32_t *ncount = nullptr;
+    // tiled layout (tiles.hpp): the current list is tiled when `tiled`
+    bool tiled = false, tile_ok = false;
+    int layout_pref = 0;  // 0: tiled when possible, until tuned; 1: tiled; 2: untiled
+    int4 *tiles = nullptr;
+    int2 *truns = nullptr;
+    int tiles_cap = 0, ntiles = 0, smax = 0;
+    size_t lds_max = 0;  // dynamic LDS a tiled workgroup may take
+    int *tinfo = nullptr, *htinfo = nullptr;
     uint32_t *keys = nullptr, *tkeys = nullptr, *hist = nullptr;
     int32_t *vals = nullptr, *tvals = nullptr, *cstart = nullptr;
     int cstart_cap = 0;
@@ -167,6 +178,14 @@ struct parm_system {
         L.type_bits = type_bits;
         L.tpa = tpa;
         L.built = built ? 1 : 0;
+        L.tiled = tiled ? 1 : 0;
+        L.ntiles = ntiles;
+        L.smax = smax;
+        L.tiles = tiles;
+        L.truns = truns;
+        L.keys = keys;
+        L.cstart = cstart;
+        L.grid = grid;
         return L;
     }
     PairView pairview() const {
@@ -288,6 +307,58 @@ int permute_all(parm_system *s, const int32_t *perm) {
     return PARM_OK;
 }
 
+// Tiles of the current cell order (tiles.hpp).  The tiled layout is used
+// when every tile's stencil fits in LDS and the tile count fits the reduction
+// scratch; otherwise (or with PARM_TILED=0) the untiled layout.
+int plan_tiles(parm_system *s) {
+    s->tiled = false;
+    s->tile_ok = false;
+    s->ntiles = 0;
+    s->smax = 0;
+    if (s->layout_pref == 2 || s->n == 0) return PARM_OK;
+    const int nrows = s->grid.ncell / s->grid.nc[0];
+    const int need = 2 * (s->n / 64) + nrows + 4;
+    if (need > s->tiles_cap) {
+        dfree(s->tiles);
+        dfree(s->truns);
+        s->tiles_cap = need;
+        PARM_CHECK(dalloc(&s->tiles, (size_t)need));
+        PARM_CHECK(dalloc(&s->truns, (size_t)need * kTileRuns));
+    }
+    PARM_HIP(launch_tiles(s->ndim, s->n, s->grid, s->keys, s->cstart, s->tiles_cap, s->tiles,
+                          s->truns, s->tinfo, s->stream));
+    PARM_HIP(hipMemcpyAsync(s->htinfo, s->tinfo, 3 * sizeof(int), hipMemcpyDeviceToHost,
+                            s->stream));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    const int nt = s->htinfo[0], smax = s->htinfo[1], bad = s->htinfo[2];
+    const int pstride = force_blocks(std::max(s->n, 1), 8);
+    if (bad || nt <= 0 || nt > pstride || tiled_build_lds(smax) > s->lds_max) return PARM_OK;
+    s->tile_ok = true;
+    s->tiled = true;
+    s->ntiles = nt;
+    s->smax = std::max(smax, 1);
+    return PARM_OK;
+}
+
+// Builds the list at positions `at` (the current ones, or xref for a
+// relayout) in the layout chosen by plan_tiles.
+int build_list(parm_system *s, const void *at) {
+    hipStream_t st = s->stream;
+    if (s->tiled) {
+        PARM_HIP(launch_build_tiled(s->ndim, s->n, s->ntiles, s->smax, s->tiles, at, s->xwrap,
+                                    s->rank, s->diam, s->keys, s->cstart, s->grid, s->box,
+                                    s->skin, s->tpa, s->cap, s->npad,
+                                    reinterpret_cast<uint16_t *>(s->nbr), s->ncount, s->xref,
+                                    s->bpart, s->stats, s->dflag + kSlotBuild, st));
+    } else {
+        PARM_HIP(launch_build_list(s->ndim, s->n, at, s->xwrap, s->rank, s->diam, s->type,
+                                   s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
+                                   s->type_bits, s->tpa, s->cap, s->npad, s->nbr, s->ncount,
+                                   s->xref, s->bpart, s->stats, s->dflag + kSlotBuild, st));
+    }
+    return PARM_OK;
+}
+
 // NeighborList::update_list's rebuild branch (src/trackers.cpp:55-69):
 // lastlocs = x, then the full pair list from the predicate.
 int rebuild(parm_system *s) {
@@ -300,6 +371,8 @@ int rebuild(parm_system *s) {
         s->cstart_cap = s->grid.ncell + 1;
         PARM_CHECK(dalloc(&s->cstart, (size_t)s->cstart_cap));
     }
+    // from here on the previous list no longer matches the atom order
+    s->built = false;
     if (n > 0) {
         PARM_HIP(launch_cell_keys(s->ndim, n, s->pos, s->rank, s->grid, s->keys, s->vals,
                                   st));
@@ -310,14 +383,11 @@ int rebuild(parm_system *s) {
     }
     PARM_HIP(launch_cell_bounds(n, s->grid.ncell, s->keys, s->cstart, st));
     if (s->cap == 0) PARM_CHECK(ensure_list_capacity(s, initial_capacity(s)));
+    PARM_CHECK(plan_tiles(s));
     for (int attempt = 0; attempt < 4; attempt++) {
         PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
         PARM_HIP(hipMemsetAsync(s->dflag + kSlotBuild, 0, sizeof(uint32_t), st));
-        PARM_HIP(launch_build_list(s->ndim, n, s->pos, s->xwrap, s->rank, s->diam, s->type,
-                                   s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
-                                   s->type_bits, s->tpa, s->cap, s->npad, s->nbr,
-                                   s->ncount, s->xref, s->bpart, s->stats,
-                                   s->dflag + kSlotBuild, st));
+        PARM_CHECK(build_list(s, s->pos));
         unsigned long long hs[2] = {0, 0};
         PARM_HIP(hipMemcpyAsync(hs, s->stats, sizeof(hs), hipMemcpyDeviceToHost, st));
         PARM_HIP(hipStreamSynchronize(st));
@@ -407,52 +477,94 @@ struct ProfScope {
 // with the cell keys and bounds of that rebuild.
 int relayout(parm_system *s) {
     if (!s->built || s->n == 0) return PARM_OK;
-    hipStream_t st = s->stream;
-    PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
-    PARM_HIP(launch_build_list(s->ndim, s->n, s->xref, s->xwrap, s->rank, s->diam, s->type,
-                               s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
-                               s->type_bits, s->tpa, s->cap, s->npad, s->nbr, s->ncount,
-                               s->xref, s->bpart, s->stats, s->dflag + kSlotBuild, st));
+    PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), s->stream));
+    PARM_CHECK(build_list(s, s->xref));
     return PARM_OK;
 }
 
-// Autotuner for the lanes-per-atom of the force kernel: each candidate lays
-// the list out for itself and times three force-only launches with events;
-// the fastest is kept for the life of the system.  Force-only launches only
-// rewrite f at the current positions, which the next step overwrites.
-int autotune(parm_system *s) {
-    if (s->tuned || s->tpa_fixed || !s->built || s->n == 0) return PARM_OK;
-    hipStream_t st = s->stream;
-    hipEvent_t e0 = pool_event(s), e1 = pool_event(s);
-    if (!e0 || !e1) return fail(PARM_E_DEVICE, "event creation failed");
-    StepCoef c{};
-    int best = s->tpa;
-    float best_ms = 1e30f;
-    for (int tpa : {1, 2, 4, 8}) {
-        s->tpa = tpa;
-        PARM_CHECK(relayout(s));
-        PARM_HIP(launch_force(s->ndim, s->prec, kStageForceOnly, false, s->arrays(),
-                              s->listview(), s->pairview(), s->box, c,
-                              s->redbuf(kSlotScratch), st));
-        float tmin = 1e30f;
-        for (int r = 0; r < 4; r++) {  // best of 4 single launches
-            PARM_HIP(hipEventRecord(e0, st));
-            PARM_HIP(launch_force(s->ndim, s->prec, kStageForceOnly, false, s->arrays(),
-                                  s->listview(), s->pairview(), s->box, c,
-                                  s->redbuf(kSlotScratch), st));
-            PARM_HIP(hipEventRecord(e1, st));
-            PARM_HIP(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            PARM_HIP(hipEventElapsedTime(&ms, e0, e1));
-            tmin = std::min(tmin, ms);
-        }
-        if (tmin < best_ms * 0.97f) {  // prefer the earlier (smaller) on ties
-            best_ms = tmin;
-            best = tpa;
+// Autotuner of the step kernel: list layout (tiled / untiled, tiles.hpp) x
+// lanes per atom.  Each candidate lays the list out for itself (same pairs:
+// relayout) and times three launches of the run's own fused step kernel
+// (`stage`), with its velocity / acceleration / force outputs redirected to
+// scratch copies and its position output in pos_alt (which the first real
+// step overwrites), so the state is untouched.  The fastest is kept for the
+// life of the system; PARM_TPA / PARM_TILED pin either choice.
+int autotune(parm_system *s, int stage, const StepCoef &c) {
+    if (s->tuned || !s->built || s->n == 0) return PARM_OK;
+    std::vector<std::pair<int, int>> cands;  // (tiled, tpa)
+    std::vector<int> layouts;
+    if (s->tile_ok && s->layout_pref != 2) layouts.push_back(1);
+    if (s->layout_pref != 1 || !s->tile_ok) layouts.push_back(0);
+    for (int lay : layouts) {
+        if (s->tpa_fixed) {
+            cands.emplace_back(lay, s->tpa);
+        } else {
+            for (int tpa : {1, 2, 4, 8}) cands.emplace_back(lay, tpa);
         }
     }
-    s->ev_used -= 2;
-    s->tpa = best;
+    if (cands.size() <= 1) {
+        if (!cands.empty() && (cands[0].first != (int)s->tiled || cands[0].second != s->tpa)) {
+            s->tiled = cands[0].first != 0;
+            s->tpa = cands[0].second;
+            PARM_CHECK(relayout(s));
+        }
+        s->tuned = true;
+        return PARM_OK;
+    }
+    hipStream_t st = s->stream;
+    const size_t n = (size_t)s->n;
+    double *sc = nullptr;
+    PARM_CHECK(dalloc(&sc, 9 * n));
+    AtomArrays A = s->arrays(true);
+    for (int k = 0; k < 3; k++) {
+        A.v[k] = sc + k * n;
+        A.a[k] = sc + (3 + k) * n;
+        A.f[k] = sc + (6 + k) * n;
+    }
+    int rc = PARM_OK;
+    for (int k = 0; k < s->ndim && rc == PARM_OK; k++) {
+        if (hipMemcpyAsync(A.v[k], s->v[k], n * sizeof(double), hipMemcpyDeviceToDevice, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(A.a[k], s->a[k], n * sizeof(double), hipMemcpyDeviceToDevice, st) !=
+                hipSuccess)
+            rc = fail(PARM_E_DEVICE, "autotune copy failed");
+    }
+    hipEvent_t e0 = pool_event(s), e1 = pool_event(s);
+    if (rc == PARM_OK && (!e0 || !e1)) rc = fail(PARM_E_DEVICE, "event creation failed");
+    size_t best = 0;
+    float best_ms = 1e30f;
+    for (size_t q = 0; q < cands.size() && rc == PARM_OK; q++) {
+        s->tiled = cands[q].first != 0;
+        s->tpa = cands[q].second;
+        if ((rc = relayout(s)) != PARM_OK) break;
+        float tmin = 1e30f;
+        for (int r = 0; r < 4 && rc == PARM_OK; r++) {  // warm-up + best of 3
+            hipError_t e = hipEventRecord(e0, st);
+            if (e == hipSuccess)
+                e = launch_force(s->ndim, s->prec, stage, false, A, s->listview(),
+                                 s->pairview(), s->box, c, s->redbuf(kSlotScratch), st);
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                rc = fail(PARM_E_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+                break;
+            }
+            if (r > 0) tmin = std::min(tmin, ms);
+        }
+        if (tmin < best_ms * 0.97f) {  // prefer the earlier candidate on ties
+            best_ms = tmin;
+            best = q;
+        }
+    }
+    if (e0 && e1) s->ev_used -= 2;
+    (void)hipStreamSynchronize(st);
+    dfree(sc);
+    if (rc != PARM_OK) return rc;
+    s->tiled = cands[best].first != 0;
+    s->tpa = cands[best].second;
+    s->layout_pref = s->tiled ? 1 : 2;
     PARM_CHECK(relayout(s));
     s->tuned = true;
     return PARM_OK;
@@ -480,10 +592,10 @@ int post_flag(parm_system *s, int slot) {
 int run_steps(parm_system *s, bool damped, const StepCoef &c, int64_t nsteps) {
     PARM_CHECK(require_ready(s));
     if (nsteps <= 0) return PARM_OK;
-    if (nsteps >= 16) PARM_CHECK(autotune(s));
-    hipStream_t st = s->stream;
     const int kick = damped ? kStageDampedKick : kStageVerletKick;
     const int kick_drift = damped ? kStageDampedKickDrift : kStageVerletKickDrift;
+    if (nsteps >= 16) PARM_CHECK(autotune(s, kick_drift, c));
+    hipStream_t st = s->stream;
     // first half of step 0
     PARM_HIP(hipMemsetAsync(s->dflag, 0, sizeof(uint32_t), st));
     PARM_HIP(launch_drift(s->ndim, kick, s->arrays(), c, s->redbuf(0), st));
@@ -595,6 +707,10 @@ void destroy(parm_system *s) {
     dfree(s->vals);
     dfree(s->tvals);
     dfree(s->cstart);
+    dfree(s->tiles);
+    dfree(s->truns);
+    dfree(s->tinfo);
+    if (s->htinfo) (void)hipHostFree(s->htinfo);
     dfree(s->stats);
     dfree(s->bpart);
     dfree(s->tab);
@@ -707,6 +823,7 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
             s->tpa_fixed = true;
         }
     }
+    if (const char *e = std::getenv("PARM_TILED")) s->layout_pref = std::atoi(e) == 0 ? 2 : 1;
     int rc = PARM_OK;
     auto bail = [&](int r) {
         destroy(s);
@@ -719,6 +836,11 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return bail(fail(PARM_E_DEVICE, std::string("device is ") + prop.gcnArchName +
                                             "; this library is built for gfx950 only"));
+    // leave room for the static LDS of the tiled kernels (stencil layout,
+    // reduction scratch)
+    s->lds_max = std::min(kTiledLdsMax, prop.sharedMemPerBlock > 8192
+                                            ? (size_t)prop.sharedMemPerBlock - 8192
+                                            : (size_t)0);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PARM_E_DEVICE, "stream creation failed"));
     if (hipEventCreateWithFlags(&s->ev_flag, hipEventDisableTiming) != hipSuccess)
@@ -749,10 +871,11 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
         (rc = dalloc(&s->bpart, 2 * (size_t)build_blocks((int)n))) ||
         (rc = dalloc(&s->part, (size_t)kRedWidth * force_blocks((int)n, 8))) ||
         (rc = dalloc(&s->out, kRedWidth)) ||
-        (rc = dalloc(&s->dflag, 2 * kFlagRing + 4)))
+        (rc = dalloc(&s->dflag, 2 * kFlagRing + 4)) || (rc = dalloc(&s->tinfo, 4)))
         return bail(rc);
     if (hipHostMalloc((void **)&s->hflag, (2 * kFlagRing + 4) * sizeof(uint32_t), 0) != hipSuccess ||
-        hipHostMalloc((void **)&s->hout, kRedWidth * sizeof(double), 0) != hipSuccess)
+        hipHostMalloc((void **)&s->hout, kRedWidth * sizeof(double), 0) != hipSuccess ||
+        hipHostMalloc((void **)&s->htinfo, 4 * sizeof(int), 0) != hipSuccess)
         return bail(fail(PARM_E_NOMEM, "pinned host allocation failed"));
     hipStream_t st = s->stream;
     const int nb = (int)((n + 255) / 256);
@@ -911,15 +1034,34 @@ int parm_nlist_pairs(parm_system *s, int64_t cap, int32_t *first, int32_t *last,
     const int n = s->n;
     std::vector<int32_t> cnt(n), orig(n), rank(n);
     std::vector<uint32_t> nb((size_t)s->cap * s->npad);
+    const uint32_t *src = s->nbr;
+    uint32_t *conv = nullptr;
+    if (s->tiled) {
+        // staged slots -> atom entries, in a temporary untiled copy
+        PARM_CHECK(dalloc(&conv, nb.size()));
+        hipError_t e = launch_tiled_to_global(s->ndim, s->ntiles, s->tiles, s->keys,
+                                              s->cstart, s->grid,
+                                              reinterpret_cast<const uint16_t *>(s->nbr),
+                                              s->ncount, s->type, s->type_bits, s->tpa,
+                                              s->cap, s->npad, conv, s->stream);
+        if (e != hipSuccess) {
+            dfree(conv);
+            return fail(PARM_E_DEVICE, std::string("pair export: ") + hipGetErrorString(e));
+        }
+        src = conv;
+    }
     PARM_HIP(hipMemcpyAsync(cnt.data(), s->ncount, sizeof(int32_t) * n,
                             hipMemcpyDeviceToHost, s->stream));
     PARM_HIP(hipMemcpyAsync(orig.data(), s->orig, sizeof(int32_t) * n,
                             hipMemcpyDeviceToHost, s->stream));
     PARM_HIP(hipMemcpyAsync(rank.data(), s->rank, sizeof(int32_t) * n,
                             hipMemcpyDeviceToHost, s->stream));
-    PARM_HIP(hipMemcpyAsync(nb.data(), s->nbr, sizeof(uint32_t) * nb.size(),
-                            hipMemcpyDeviceToHost, s->stream));
-    PARM_HIP(hipStreamSynchronize(s->stream));
+    hipError_t ec = hipMemcpyAsync(nb.data(), src, sizeof(uint32_t) * nb.size(),
+                                   hipMemcpyDeviceToHost, s->stream);
+    if (ec == hipSuccess) ec = hipStreamSynchronize(s->stream);
+    dfree(conv);
+    if (ec != hipSuccess)
+        return fail(PARM_E_DEVICE, std::string("pair export: ") + hipGetErrorString(ec));
     std::vector<int32_t> slot_of_rank(s->nmem, -1);
     for (int i = 0; i < n; i++)
         if (rank[i] >= 0) slot_of_rank[rank[i]] = i;

@@ -43,6 +43,26 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                              unsigned long long *stats, uint32_t *flag,
                              hipStream_t s);
 
+// Tiled layout (tiles.hpp).  launch_tiles fills tiles (cap_tiles records)
+// and info[0] = tiles, info[1] = largest staged stencil, info[2] != 0 when the
+// layout does not apply.
+hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys,
+                        const int32_t *cstart, int cap_tiles, int4 *tiles, int2 *truns,
+                        int *info, hipStream_t s);
+size_t tiled_build_lds(int smax);
+hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const int4 *tiles,
+                              const void *pos, void *xw, const int32_t *rank,
+                              const double *diam, const uint32_t *keys,
+                              const int32_t *cstart, const CellGrid &g, const Box &box,
+                              double skin, int tpa, int cap, int npad, uint16_t *nb16,
+                              int32_t *ncount, void *xref, unsigned long long *part,
+                              unsigned long long *stats, uint32_t *flag, hipStream_t s);
+hipError_t launch_tiled_to_global(int ndim, int ntiles, const int4 *tiles,
+                                  const uint32_t *keys, const int32_t *cstart,
+                                  const CellGrid &g, const uint16_t *nb16,
+                                  const int32_t *ncount, const int32_t *type, int type_bits,
+                                  int tpa, int cap, int npad, uint32_t *out, hipStream_t s);
+
 // -------------------------------------------------------------- step.hip
 enum PairKind : int { kPairNone = 0, kPairLJ = 1, kPairRepulsion = 2 };
 enum Precision : int { kFP64 = 0, kMixed = 1 };
@@ -82,6 +102,15 @@ struct ListView {
     int type_bits;
     int tpa;               // lanes per atom; entries grouped by tpa (see k_force)
     int built;             // 0: the list was never built -> no pairs
+    // tiled layout (tiles.hpp): nbr holds 16-bit staged slots
+    int tiled;
+    int ntiles;            // tiles = workgroups of a tiled force launch
+    int smax;              // largest staged stencil of any tile
+    const int4 *tiles;
+    const int2 *truns;     // kTileRuns runs per tile (tiles.hpp)
+    const uint32_t *keys;  // sorted cell keys of the last rebuild
+    const int32_t *cstart;
+    CellGrid grid;
 };
 
 struct PairView {
@@ -123,6 +152,14 @@ hipError_t launch_force(int ndim, int precision, int stage, bool energy,
                         const AtomArrays &A, const ListView &Lv, const PairView &P,
                         const Box &box, const StepCoef &c, const ReduceBuf &R,
                         hipStream_t s);
+// Tiled force launch (step_tiled.hip); same contract as launch_force.
+hipError_t launch_force_tiled(int ndim, int precision, int stage, bool energy,
+                              const AtomArrays &A, const ListView &Lv, const PairView &P,
+                              const Box &box, const StepCoef &c, const ReduceBuf &R,
+                              hipStream_t s);
+// k_finalize over nparts workgroup partials.
+hipError_t launch_finalize_rt(bool sums, bool skin, int nparts, const ReduceBuf &R,
+                              hipStream_t s);
 hipError_t launch_drift(int ndim, int stage, const AtomArrays &A, const StepCoef &c,
                         const ReduceBuf &R, hipStream_t s);
 hipError_t launch_skin(int ndim, const AtomArrays &A, const ReduceBuf &R,

@@ -14,8 +14,15 @@
 //   gathers          every per-slot array permuted into the new order
 //   k_cell_bounds    first slot of each cell
 //   k_build_list<D>  full (both directions) list, ELL layout nbr[k*npad + i]
+//
+// Tiled layout (tiles.hpp), used when every tile's stencil fits in LDS:
+//   k_tile_rows      rows of cells -> tiles of <= kTileAtoms atoms
+//   k_tile_sizes<D>  staged slots of each tile, their maximum
+//   k_build_tiled<D> the tile's stencil staged in LDS, the same predicate
+//                    evaluated from LDS, 16-bit staged-slot entries
 #include "common.hpp"
 #include "kernels.hpp"
+#include "tiles.hpp"
 
 namespace parm {
 
@@ -381,6 +388,321 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
     }
 }
 
+// ------------------------------------------------------------ row tiles
+// One workgroup of 1024 threads.  Row r (cells r*nc0 .. r*nc0 + nc0 - 1)
+// holds atoms cstart[r*nc0] .. cstart[(r+1)*nc0]; it is cut into
+// ceil(n_r / kTileAtoms) tiles in order.  Non-members (slots from
+// cstart[ncell] on) get tail tiles.  info[0] = number of tiles (0 and
+// info[2] |= 1 if they exceed cap_tiles).
+constexpr int kTileScan = 1024;
+
+__device__ __forceinline__ int block_excl_scan(int v, int &total) {
+    __shared__ int wsum[kTileScan / kWave];
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int q = 0; q < kTileScan / kWave; q++) {
+        const int ws = wsum[q];
+        if (q < w) before += ws;
+        total += ws;
+    }
+    __syncthreads();
+    return before + incl - v;
+}
+
+__global__ __launch_bounds__(kTileScan) void k_tile_rows(
+    int n, int nrows, int nc0, int ncell, const int32_t *__restrict__ cstart,
+    int cap_tiles, TileRec *__restrict__ tiles, int *__restrict__ info) {
+    int base = 0;
+    for (int r0 = 0; r0 < nrows; r0 += kTileScan) {
+        const int r = r0 + (int)threadIdx.x;
+        int a = 0, cnt = 0, nt = 0;
+        if (r < nrows) {
+            a = cstart[r * nc0];
+            cnt = cstart[(r + 1) * nc0] - a;
+            nt = (cnt + kTileAtoms - 1) / kTileAtoms;
+        }
+        int total = 0;
+        const int off = base + block_excl_scan(nt, total);
+        for (int q = 0; q < nt; q++) {
+            const int idx = off + q;
+            if (idx < cap_tiles)
+                tiles[idx] = TileRec{a + q * kTileAtoms,
+                                     min(kTileAtoms, cnt - q * kTileAtoms), r, 0};
+        }
+        base += total;
+    }
+    const int m0 = cstart[ncell];
+    const int ntail = (n - m0 + kTileAtoms - 1) / kTileAtoms;
+    for (int q = (int)threadIdx.x; q < ntail; q += kTileScan) {
+        const int idx = base + q;
+        if (idx < cap_tiles)
+            tiles[idx] = TileRec{m0 + q * kTileAtoms, min(kTileAtoms, n - m0 - q * kTileAtoms),
+                                 -1, 0};
+    }
+    if (threadIdx.x == 0) {
+        const int nt = base + ntail;
+        info[0] = nt <= cap_tiles ? nt : 0;
+        info[1] = 0;
+        info[2] = nt <= cap_tiles ? 0 : 1;
+    }
+}
+
+// One thread per tile: stencil width and staged slots.  A tile whose stencil
+// is wider than kTileWMax cells or holds more than kTileMaxSlots atoms marks
+// info[2]; the system then uses the untiled layout.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_tile_sizes(int cap_tiles, CellGrid g,
+                                                       const uint32_t *__restrict__ keys,
+                                                       const int32_t *__restrict__ cstart,
+                                                       TileRec *__restrict__ tiles,
+                                                       int2 *__restrict__ truns,
+                                                       int *__restrict__ info) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int nt = info[0];
+    int S = 0, bad = 0;
+    if (t < nt && t < cap_tiles) {
+        TileRec T = tiles[t];
+        int2 *runs = truns + (size_t)t * kTileRuns;
+        int nr = 0;
+        if (T.z >= 0 && T.y > 0) {
+            int xa = (int)(keys[T.x] % (uint32_t)g.nc[0]);
+            const int xb = (int)(keys[T.x + T.y - 1] % (uint32_t)g.nc[0]);
+            int W = 0;
+            const int xfull = stencil_x(g, xa, xb, W);
+            if (W > kTileWMax) {
+                bad = 1;
+            } else {
+                const int cy = D >= 2 ? T.z % g.nc[1] : 0;
+                const int cz = D == 3 ? T.z / g.nc[1] : 0;
+                const int NR = stencil_ny<D>(g) * stencil_nz<D>(g);
+                // per stencil row: the cells before / after the wrap along x
+                int ustar = W;
+                if (g.nc[0] >= 3 && !xfull) {
+                    if (xa - 1 < 0)
+                        ustar = 1;
+                    else if (xa - 1 + W > g.nc[0])
+                        ustar = g.nc[0] - (xa - 1);
+                }
+                for (int k = 0; k < NR; k++) {
+                    for (int h = 0; h < 2; h++) {
+                        const int u0 = h == 0 ? 0 : ustar, u1 = h == 0 ? ustar : W;
+                        int g0 = 0, len = 0;
+                        if (u1 > u0) {
+                            int img[3];
+                            const int c0 = stencil_cell<D>(g, cy, cz, xa, k, u0, img);
+                            const int c1 = stencil_cell<D>(g, cy, cz, xa, k, u1 - 1, img);
+                            g0 = cstart[c0];
+                            len = cstart[c1 + 1] - g0;
+                        }
+                        runs[nr++] = int2{g0, S};
+                        S += len;
+                    }
+                }
+                if (S > kTileMaxSlots) bad = 1;
+            }
+        }
+        for (; nr < kTileRuns; nr++) runs[nr] = int2{0, S};
+        T.w = bad ? -1 : S;
+        tiles[t] = T;
+    }
+    // one atomic per wavefront
+    int smax = bad ? 0 : S;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        smax = max(smax, __shfl_xor(smax, o));
+        bad |= __shfl_xor(bad, o);
+    }
+    if (threadIdx.x % kWave == 0) {
+        if (smax > 0) atomicMax(info + 1, smax);
+        if (bad) atomicOr(info + 2, 1);
+    }
+}
+
+// Tiled list build: one workgroup per tile, G lanes per atom.  The tile's
+// stencil is staged in LDS as wrapped positions shifted by the stencil
+// cell's periodic image (so |w_i - staged| is the minimum-image distance up
+// to rounding) with the list diameter, plus the atom index for the exact
+// test.  Each atom then scans one contiguous staged range per stencil row.
+// The predicate is the one k_build_list evaluates (1e-6 prefilter band, the
+// reference's exact test inside it); entries are staged slots.
+template <int D, int G, int U>
+__global__ __launch_bounds__(kBlock) void k_build_tiled(
+    const TileRec *__restrict__ tiles, int smax, const typename PosStore<D>::T *__restrict__ pos,
+    const typename PosStore<D>::T *__restrict__ xw, const int32_t *__restrict__ rank,
+    const double *__restrict__ diam, const uint32_t *__restrict__ keys,
+    const int32_t *__restrict__ cstart, CellGrid g, Box box, double skin, int tpa, int cap,
+    int npad, uint16_t *__restrict__ nb16, int32_t *__restrict__ ncount,
+    typename PosStore<D>::T *__restrict__ xref, unsigned long long *__restrict__ part,
+    uint32_t *__restrict__ flag) {
+    extern __shared__ double4 st_pos[];
+    __shared__ TileLayout L;
+    __shared__ unsigned long long red[kBlock / kWave][2];
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    tile_layout<D>(tiles, t, keys, cstart, g, L);
+    const int S = L.off[L.NR * L.W];
+    for (int s = threadIdx.x; s < S; s += kBlock) {
+        const int c = slot_cell(L, s);
+        int img[3];
+        (void)stencil_cell<D>(g, L.cy, L.cz, L.xa, c / L.W, c % L.W, img);
+        const int j = L.g0[c] + (s - L.off[c]);
+        const Vec<D> w = PosStore<D>::load(xw[j]);
+        double4 v;
+        v.x = w.c[0] + img[0] * box.L[0];
+        v.y = D >= 2 ? w.c[1] + img[1] * box.L[1] : 0.0;
+        if (D == 3) {
+            v.z = w.c[2] + img[2] * box.L[2];
+            v.w = diam[j];
+        } else {
+            v.z = diam[j];
+            v.w = 0.0;
+        }
+        st_pos[s] = v;
+    }
+    __syncthreads();
+    const int li = threadIdx.x / G, sub = threadIdx.x % G;
+    const int lane = threadIdx.x % kWave;
+    const int gbase = lane - sub;
+    const unsigned long long below = (sub == 0) ? 0ull : ((1ull << sub) - 1ull);
+    const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    unsigned long long mycnt = 0;
+    if (li < L.cnt) {
+        const int i = L.a0 + li;
+        const typename PosStore<D>::T pi = pos[i];
+        if (sub == 0) xref[i] = pi;
+        if (L.row < 0 || rank[i] < 0) {
+            if (sub == 0) ncount[i] = 0;
+        } else {
+            const Vec<D> xi = PosStore<D>::load(pi);
+            const Vec<D> wi = PosStore<D>::load(xw[i]);
+            const double di = diam[i];
+            const int cx = (int)(keys[i] % (uint32_t)g.nc[0]);
+            // atom i's own staged slot (centre stencil row, its cell)
+            int self = -1;
+            {
+                const int kc = (stencil_ny<D>(g) == 3 ? 1 : 0) +
+                               (stencil_nz<D>(g) == 3 ? stencil_ny<D>(g) : 0);
+                const int uc = g.nc[0] < 3 ? 0 : (L.xfull ? cx : cx - L.xa + 1);
+                const int cc = kc * L.W + uc;
+                self = L.off[cc] + (i - L.g0[cc]);
+            }
+            // along x the minimum image is explicit for a collapsed
+            // dimension or a full-row stencil (no staged shift)
+            const bool fold0 = g.nc[0] < 3 || L.xfull;
+            int cnt = 0;
+            for (int kr = 0; kr < 2 * L.NR; kr++) {
+                int lo, hi, lo1, hi1;
+                stencil_ranges(L, g, kr >> 1, cx, lo, hi, lo1, hi1);
+                if (kr & 1) {
+                    lo = lo1;
+                    hi = hi1;
+                }
+                for (int base = lo; base < hi; base += G * U) {
+                    double4 pw[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int s = base + u * G + sub;
+                        pw[u] = st_pos[s < hi ? s : lo];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int s = base + u * G + sub;
+                        bool pass = false;
+                        if (s < hi && s != self) {
+                            {
+                                const double pc[3] = {pw[u].x, pw[u].y, pw[u].z};
+                                const double dj = D == 3 ? pw[u].w : pw[u].z;
+                                double d2w = 0.0;
+#pragma unroll
+                                for (int k2 = 0; k2 < D; k2++) {
+                                    double dk = wi.c[k2] - pc[k2];
+                                    if (k2 == 0 ? fold0 : g.nc[k2] < 3)
+                                        dk -= box.L[k2] * rint(dk * box.invL[k2]);
+                                    d2w = fma(dk, dk, d2w);
+                                }
+                                const double cc = (di + dj) / 2 + skin;
+                                const double c2 = cc * cc;
+                                if (d2w < c2 * (1.0 - 1e-6))
+                                    pass = true;
+                                else if (d2w <= c2 * (1.0 + 1e-6)) {
+                                    const int cj = slot_cell(L, s);
+                                    const int j = L.g0[cj] + (s - L.off[cj]);
+                                    pass = in_list<D>(xi, PosStore<D>::load(pos[j]), di, dj,
+                                                      box, skin);
+                                }
+                            }
+                        }
+                        const unsigned long long m = (__ballot(pass) >> gbase) & gmask;
+                        if (pass) {
+                            const int q = cnt + __popcll(m & below);
+                            if (q < cap) nb16[tiled_entry(q, i, tpa, npad)] = (uint16_t)s;
+                        }
+                        cnt += __popcll(m);
+                    }
+                }
+            }
+            if (sub == 0) {
+                ncount[i] = cnt < cap ? cnt : cap;
+                if (cnt > cap) atomicOr(flag, kFlagOverflow);
+                mycnt = (unsigned long long)cnt;
+            }
+        }
+    }
+    unsigned long long sum = mycnt, mx = mycnt;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off);
+        const unsigned long long o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if (lane == 0) {
+        red[threadIdx.x / kWave][0] = sum;
+        red[threadIdx.x / kWave][1] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s2 = 0, m2 = 0;
+        for (int q = 0; q < kBlock / kWave; q++) {
+            s2 += red[q][0];
+            m2 = red[q][1] > m2 ? red[q][1] : m2;
+        }
+        part[2 * (size_t)blockIdx.x] = s2;
+        part[2 * (size_t)blockIdx.x + 1] = m2;
+    }
+}
+
+// Tiled list -> untiled entries ((j << type_bits) | type_j, grouped by tpa),
+// for exporting the pairs.  One workgroup per tile.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_tiled_to_global(
+    const TileRec *__restrict__ tiles, const uint32_t *__restrict__ keys,
+    const int32_t *__restrict__ cstart, CellGrid g, const uint16_t *__restrict__ nb16,
+    const int32_t *__restrict__ ncount, const int32_t *__restrict__ type, int type_bits,
+    int tpa, int cap, int npad, uint32_t *__restrict__ out) {
+    __shared__ TileLayout L;
+    const int t = blockIdx.x;
+    tile_layout<D>(tiles, t, keys, cstart, g, L);
+    if (L.row < 0) return;
+    for (int e = threadIdx.x; e < L.cnt * cap; e += kBlock) {
+        const int li = e % L.cnt, q = e / L.cnt;
+        const int i = L.a0 + li;
+        if (q >= ncount[i]) continue;
+        const int s = nb16[tiled_entry(q, i, tpa, npad)];
+        const int c = slot_cell(L, s);
+        const int j = L.g0[c] + (s - L.off[c]);
+        out[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] =
+            ((uint32_t)j << type_bits) | (uint32_t)type[j];
+    }
+}
+
 // Wrapped copy of the positions, frac(x / L) * L per component, computed
 // with the same floor as the cell keys: the list build's prefilter uses it.
 template <int D>
@@ -540,6 +862,74 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                            nbr, ncount, (double2 *)xref, part, flag);
     }
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, nb, part, stats);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ tiled launchers
+hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys,
+                        const int32_t *cstart, int cap_tiles, TileRec *tiles, int2 *truns,
+                        int *info, hipStream_t s) {
+    const int nrows = g.ncell / g.nc[0];
+    hipLaunchKernelGGL(k_tile_rows, dim3(1), dim3(kTileScan), 0, s, n, nrows, g.nc[0],
+                       g.ncell, cstart, cap_tiles, tiles, info);
+    const int nb = (cap_tiles + kBlock - 1) / kBlock;
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_tile_sizes<3>, dim3(nb), dim3(kBlock), 0, s, cap_tiles, g, keys,
+                           cstart, tiles, truns, info);
+    else
+        hipLaunchKernelGGL(k_tile_sizes<2>, dim3(nb), dim3(kBlock), 0, s, cap_tiles, g, keys,
+                           cstart, tiles, truns, info);
+    return hipGetLastError();
+}
+
+constexpr int kTiledLanes = 4;   // lanes per atom in k_build_tiled
+constexpr int kTiledUnroll = 4;
+
+size_t tiled_build_lds(int smax) { return (size_t)smax * sizeof(double4); }
+
+hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileRec *tiles,
+                              const void *pos, void *xw, const int32_t *rank,
+                              const double *diam, const uint32_t *keys,
+                              const int32_t *cstart, const CellGrid &g, const Box &box,
+                              double skin, int tpa, int cap, int npad, uint16_t *nb16,
+                              int32_t *ncount, void *xref, unsigned long long *part,
+                              unsigned long long *stats, uint32_t *flag, hipStream_t s) {
+    static_assert(kTiledLanes * kTileAtoms == kBlock, "one tile per workgroup");
+    if (n <= 0 || ntiles <= 0) return hipSuccess;
+    const size_t lds = tiled_build_lds(smax);
+    if (ndim == 3) {
+        hipLaunchKernelGGL(k_wrap<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
+                           (const double4 *)pos, g, box, (double4 *)xw);
+        hipLaunchKernelGGL((k_build_tiled<3, kTiledLanes, kTiledUnroll>), dim3(ntiles),
+                           dim3(kBlock), lds, s, tiles, smax, (const double4 *)pos,
+                           (const double4 *)xw, rank, diam, keys, cstart, g, box, skin, tpa,
+                           cap, npad, nb16, ncount, (double4 *)xref, part, flag);
+    } else {
+        hipLaunchKernelGGL(k_wrap<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
+                           (const double2 *)pos, g, box, (double2 *)xw);
+        hipLaunchKernelGGL((k_build_tiled<2, kTiledLanes, kTiledUnroll>), dim3(ntiles),
+                           dim3(kBlock), lds, s, tiles, smax, (const double2 *)pos,
+                           (const double2 *)xw, rank, diam, keys, cstart, g, box, skin, tpa,
+                           cap, npad, nb16, ncount, (double2 *)xref, part, flag);
+    }
+    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, ntiles, part, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_tiled_to_global(int ndim, int ntiles, const TileRec *tiles,
+                                  const uint32_t *keys, const int32_t *cstart,
+                                  const CellGrid &g, const uint16_t *nb16,
+                                  const int32_t *ncount, const int32_t *type, int type_bits,
+                                  int tpa, int cap, int npad, uint32_t *out, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_tiled_to_global<3>, dim3(ntiles), dim3(kBlock), 0, s, tiles,
+                           keys, cstart, g, nb16, ncount, type, type_bits, tpa, cap, npad,
+                           out);
+    else
+        hipLaunchKernelGGL(k_tiled_to_global<2>, dim3(ntiles), dim3(kBlock), 0, s, tiles,
+                           keys, cstart, g, nb16, ncount, type, type_bits, tpa, cap, npad,
+                           out);
     return hipGetLastError();
 }
 

@@ -80,23 +80,20 @@ __device__ __forceinline__ void rep64(const PairParam &pp, const Vec<D> &r,
     }
 }
 
-// Mixed precision: the displacement is formed in FP64 (exact minimum image),
-// then r^2, r^-6 and the force are evaluated in FP32.
-template <int D, bool ENERGY>
-__device__ __forceinline__ void lj32(const PairParam &pp, const Vec<D> &r,
-                                     float (&f)[D], PairAcc &acc) {
-    float rf[D];
-#pragma unroll
-    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
+// Mixed precision: r^2, r^-6 and the force are evaluated in FP32 from an FP32
+// displacement rf.  The in/out decision near the cutoff must not flip with
+// FP32 rounding (the LJ force is not shifted to zero there): within 1e-5 of
+// the cutoff it is taken in FP64 exactly as the FP64 path takes it, on the
+// d2 returned by exact_d2() (the FP64 minimum-image displacement).
+template <int D, bool ENERGY, typename Exact>
+__device__ __forceinline__ void lj32f(const PairParam &pp, const float (&rf)[D],
+                                      const Exact &exact_d2, float (&f)[D], PairAcc &acc) {
     float d2 = rf[0] * rf[0];
 #pragma unroll
     for (int k = 1; k < D; k++) d2 = fmaf(rf[k], rf[k], d2);
-    // The LJ force is not shifted to zero at the cutoff, so the in/out
-    // decision must not flip with FP32 rounding: within 1e-5 of the cutoff it
-    // is taken in FP64 exactly as the FP64 path takes it.
     if (d2 > pp.lim_f * 0.99999f) {
         if (d2 > pp.lim_f * 1.00001f) return;
-        const double d2d = sqnorm<D>(r);
+        const double d2d = exact_d2();
         if (!(d2d < pp.lim_lo || (!(d2d > pp.lim_hi) && !(d2d / pp.sig2 > pp.cut2))))
             return;
     }
@@ -122,18 +119,15 @@ __device__ __forceinline__ void lj32(const PairParam &pp, const Vec<D> &r,
     }
 }
 
-template <int D, bool ENERGY, bool HARM>
-__device__ __forceinline__ void rep32(const PairParam &pp, const Vec<D> &r,
-                                      float (&f)[D], PairAcc &acc) {
-    float rf[D];
-#pragma unroll
-    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
+template <int D, bool ENERGY, bool HARM, typename Exact>
+__device__ __forceinline__ void rep32f(const PairParam &pp, const float (&rf)[D],
+                                       const Exact &exact_d2, float (&f)[D], PairAcc &acc) {
     float d2 = rf[0] * rf[0];
 #pragma unroll
     for (int k = 1; k < D; k++) d2 = fmaf(rf[k], rf[k], d2);
     if (d2 > pp.sig2_f * 0.99999f) {  // contact decided in FP64 near sigma
         if (d2 > pp.sig2_f * 1.00001f) return;
-        if (sqnorm<D>(r) > pp.sig2) return;
+        if (exact_d2() > pp.sig2) return;
     }
     const float R = __builtin_sqrtf(d2);
     const float x = fmaxf(1.0f - R * pp.isig_f, 0.0f);
@@ -157,6 +151,26 @@ __device__ __forceinline__ void rep32(const PairParam &pp, const Vec<D> &r,
         acc.contacts += (e != 0.0) ? 1.0 : 0.0;
         acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
     }
+}
+
+// Untiled mixed path: the displacement is formed in FP64 (exact minimum
+// image) and rounded to FP32.
+template <int D, bool ENERGY>
+__device__ __forceinline__ void lj32(const PairParam &pp, const Vec<D> &r,
+                                     float (&f)[D], PairAcc &acc) {
+    float rf[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
+    lj32f<D, ENERGY>(pp, rf, [&]() { return sqnorm<D>(r); }, f, acc);
+}
+
+template <int D, bool ENERGY, bool HARM>
+__device__ __forceinline__ void rep32(const PairParam &pp, const Vec<D> &r,
+                                      float (&f)[D], PairAcc &acc) {
+    float rf[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
+    rep32f<D, ENERGY, HARM>(pp, rf, [&]() { return sqnorm<D>(r); }, f, acc);
 }
 
 // ------------------------------------------------------------ step kernels

@@ -20,67 +20,9 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "pairs.hpp"
+#include "stepcore.hpp"
 
 namespace parm {
-
-// ------------------------------------------------------ reductions
-// A kernel that reduces (sums of energy / virial / stress / counts, and the
-// skin test's top-2 and non-finite count) writes one partial per workgroup
-// with plain stores; k_finalize, launched right after it on the same stream,
-// reduces the partials in a fixed order.  No fences or contended atomics are
-// needed inside the hot kernel, and the result does not depend on scheduling.
-constexpr int kSumU = 0, kSumW = 1, kSumS = 2, kSumContacts = 11,
-              kSumOverlaps = 12, kTop1 = 13, kTop2 = 14, kNonFinite = 15;
-
-template <bool SUMS, bool SKIN>
-__device__ __forceinline__ void block_partial(double (&v)[kRedWidth], Top2 t,
-                                              const ReduceBuf &R) {
-    __shared__ double sh[kBlock / kWave][kRedWidth];
-    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-    if (SUMS) {
-#pragma unroll
-        for (int k = 0; k < kTop1; k++) v[k] = wave_sum(v[k]);
-    }
-    if (SKIN) {
-        t = wave_top2(t);
-        v[kNonFinite] = wave_sum(v[kNonFinite]);
-    }
-    if (lane == 0) {
-        if (SUMS) {
-#pragma unroll
-            for (int k = 0; k < kTop1; k++) sh[w][k] = v[k];
-        }
-        if (SKIN) {
-            sh[w][kTop1] = t.a;
-            sh[w][kTop2] = t.b;
-            sh[w][kNonFinite] = v[kNonFinite];
-        }
-    }
-    __syncthreads();
-    const int k = threadIdx.x;
-    // field-major partials: field k of workgroup b at part[k * pstride + b]
-    double *dst = R.part + blockIdx.x;
-    const size_t ps = (size_t)R.pstride;
-    if (SUMS && k < kTop1) {
-        double p = 0.0;
-        for (int q = 0; q < kBlock / kWave; q++) p += sh[q][k];
-        dst[k * ps] = p;
-    }
-    if (SKIN && k == kTop1) {
-        Top2 bt;
-        double nf = 0.0;
-        for (int q = 0; q < kBlock / kWave; q++) {
-            Top2 o;
-            o.a = sh[q][kTop1];
-            o.b = sh[q][kTop2];
-            bt.merge(o);
-            nf += sh[q][kNonFinite];
-        }
-        dst[kTop1 * ps] = bt.a;
-        dst[kTop2 * ps] = bt.b;
-        dst[kNonFinite * ps] = nf;
-    }
-}
 
 // One workgroup of kFinal threads: thread t reduces partials t, t + kFinal,
 // ... of each field (Neumaier; the loads of a field are coalesced across the
@@ -179,19 +121,6 @@ static void launch_finalize(int nparts, const ReduceBuf &R, hipStream_t s) {
     hipLaunchKernelGGL((k_finalize<SUMS, SKIN>), dim3(1), dim3(kFinal), 0, s, nparts, R);
 }
 
-template <int D>
-__device__ __forceinline__ void skin_atom(const Vec<D> &x, const void *xref, int i,
-                                          Top2 &t, double &nonfinite) {
-    const Vec<D> x0 =
-        PosStore<D>::load(static_cast<const typename PosStore<D>::T *>(xref)[i]);
-    Vec<D> d;
-#pragma unroll
-    for (int k = 0; k < D; k++) d.c[k] = x.c[k] - x0.c[k];
-    const double dist = sqrt(sqnorm<D>(d));
-    t.add(dist);
-    if (!isfinite(dist)) nonfinite += 1.0;
-}
-
 // TPA lanes share one atom: lane `sub` visits list entries sub, sub + TPA, ...
 // The list is stored in groups of TPA entries per atom (entry q of slot i at
 // ((q / TPA) * npad + i) * TPA + q % TPA), so the 64 lanes of a wave read
@@ -205,17 +134,11 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
                                                   PairView P, Box box, StepCoef c,
                                                   ReduceBuf R) {
     using PT = typename PosStore<D>::T;
-    constexpr bool DRIFT =
-        (STAGE == kStageVerletKickDrift || STAGE == kStageDampedKickDrift);
-    constexpr bool DAMPED =
-        (STAGE == kStageDampedKick || STAGE == kStageDampedKickDrift);
     const int gt = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const int i = gt / TPA;
     const int sub = gt % TPA;
     const bool act = i < A.n;
     PairAcc acc;
-    Top2 top;
-    double nonfinite = 0.0;
     double fi[D];
     float f32[D];
 #pragma unroll
@@ -272,58 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
         for (int k = 0; k < D; k++) fi[k] += __shfl_xor(fi[k], o);
     }
 
-    if (act && sub == 0 && STAGE != kStageSums) {
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            if (STAGE == kStageForceAdd)
-                A.f[k][i] += fi[k];
-            else
-                A.f[k][i] = fi[k];
-        }
-        if (STAGE != kStageForceOnly && STAGE != kStageForceAdd) {
-            const double m = A.m[i];
-            const bool fixed = is_fixed(m);
-            double a[D];
-#pragma unroll
-            for (int k = 0; k < D; k++) a[k] = fixed ? 0.0 : fi[k] / m;
-#pragma unroll
-            for (int k = 0; k < D; k++) A.a[k][i] = a[k];
-            if (STAGE != kStageForceA) {
-                double v[D];
-#pragma unroll
-                for (int k = 0; k < D; k++) v[k] = A.v[k][i];
-                if (!fixed) {
-#pragma unroll
-                    for (int k = 0; k < D; k++)
-                        v[k] = v[k] + a[k] * (DAMPED ? c.k4 : c.k2);
-                }
-                if (DRIFT) {
-                    Vec<D> xn = xi;
-                    drift_atom<D, DAMPED>(xn, v, a, fixed, c);
-                    static_cast<PT *>(A.pos_out)[i] = PosStore<D>::make(xn);
-                    if (A.rank[i] >= 0) skin_atom<D>(xn, A.xref, i, top, nonfinite);
-                }
-#pragma unroll
-                for (int k = 0; k < D; k++) A.v[k][i] = v[k];
-            }
-        }
-    }
-
-    if (ENERGY || DRIFT) {
-        double v[kRedWidth];
-#pragma unroll
-        for (int k = 0; k < kRedWidth; k++) v[k] = 0.0;
-        if (ENERGY) {
-            v[kSumU] = 0.5 * acc.e;
-            v[kSumW] = 0.5 * acc.w;
-#pragma unroll
-            for (int k = 0; k < D * D; k++) v[kSumS + k] = 0.5 * acc.s[k];
-            v[kSumContacts] = acc.contacts;
-            v[kSumOverlaps] = acc.overlaps;
-        }
-        v[kNonFinite] = nonfinite;
-        block_partial<ENERGY, DRIFT>(v, top, R);
-    }
+    atom_finish<D, STAGE, ENERGY, kBlock / kWave>(A, i, act, sub, fi, xi, c, acc, R);
 }
 
 // Standalone first half of a step (drift + half kick) and skin test.
@@ -354,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_drift(AtomArrays A, StepCoef c,
 #pragma unroll
     for (int k = 0; k < kRedWidth; k++) v[k] = 0.0;
     v[kNonFinite] = nonfinite;
-    block_partial<false, true>(v, top, R);
+    block_partial<false, true, kBlock / kWave>(v, top, R);
 }
 
 // Skin test only (after a rebuild, or for update_list(false) on demand).
@@ -372,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_skin(AtomArrays A, ReduceBuf R) {
 #pragma unroll
     for (int k = 0; k < kRedWidth; k++) v[k] = 0.0;
     v[kNonFinite] = nonfinite;
-    block_partial<false, true>(v, top, R);
+    block_partial<false, true, kBlock / kWave>(v, top, R);
 }
 
 // Atom-wise observables (AtomGroup::kinetic_energy / momentum / mass,
@@ -413,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_atom_sums(AtomArrays A, Vec<D> ov,
             top.add(fn);
         }
     }
-    block_partial<true, MODE == 1>(v, top, R);
+    block_partial<true, MODE == 1, kBlock / kWave>(v, top, R);
 }
 
 template <int D>
@@ -465,17 +337,14 @@ static void launch_force_tt(bool energy, const AtomArrays &A, const ListView &Lv
                             const ReduceBuf &R, hipStream_t s) {
     constexpr bool DRIFT =
         (STAGE == kStageVerletKickDrift || STAGE == kStageDampedKickDrift);
+    // energies are accumulated by the sums stage only
+    constexpr bool ENERGY = STAGE == kStageSums;
+    (void)energy;
     const int nb = force_blocks(A.n, TPA);
     dim3 g(nb), b(kBlock);
-    if (energy) {
-        hipLaunchKernelGGL((k_force<D, KIND, PREC, SINGLE, HARM, STAGE, true, TPA>), g,
-                           b, 0, s, A, Lv, P, box, c, R);
-        launch_finalize<true, DRIFT>(nb, R, s);
-    } else {
-        hipLaunchKernelGGL((k_force<D, KIND, PREC, SINGLE, HARM, STAGE, false, TPA>), g,
-                           b, 0, s, A, Lv, P, box, c, R);
-        if (DRIFT) launch_finalize<false, true>(nb, R, s);
-    }
+    hipLaunchKernelGGL((k_force<D, KIND, PREC, SINGLE, HARM, STAGE, ENERGY, TPA>), g, b, 0,
+                       s, A, Lv, P, box, c, R);
+    if (ENERGY || DRIFT) launch_finalize<ENERGY, DRIFT>(nb, R, s);
 }
 
 template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE>
@@ -528,14 +397,18 @@ template <int D, int KIND, int PREC>
 static void launch_force_k(int stage, bool energy, const AtomArrays &A,
                            const ListView &Lv, const PairView &P, const Box &box,
                            const StepCoef &c, const ReduceBuf &R, hipStream_t s) {
-    if (KIND == kPairRepulsion && P.harmonic) {
-        if (P.single)
-            launch_force_s<D, KIND, PREC, true, true>(stage, energy, A, Lv, P, box, c,
-                                                      R, s);
-        else
-            launch_force_s<D, KIND, PREC, false, true>(stage, energy, A, Lv, P, box,
-                                                       c, R, s);
-    } else {
+    if constexpr (KIND == kPairRepulsion) {
+        if (P.harmonic) {
+            if (P.single)
+                launch_force_s<D, KIND, PREC, true, true>(stage, energy, A, Lv, P, box,
+                                                          c, R, s);
+            else
+                launch_force_s<D, KIND, PREC, false, true>(stage, energy, A, Lv, P, box,
+                                                           c, R, s);
+            return;
+        }
+    }
+    {
         if (P.single)
             launch_force_s<D, KIND, PREC, true, false>(stage, energy, A, Lv, P, box,
                                                        c, R, s);
@@ -565,11 +438,25 @@ static void launch_force_d(int precision, int stage, bool energy, const AtomArra
     }
 }
 
+hipError_t launch_finalize_rt(bool sums, bool skin, int nparts, const ReduceBuf &R,
+                              hipStream_t s) {
+    if (sums && skin)
+        launch_finalize<true, true>(nparts, R, s);
+    else if (sums)
+        launch_finalize<true, false>(nparts, R, s);
+    else if (skin)
+        launch_finalize<false, true>(nparts, R, s);
+    return hipGetLastError();
+}
+
 hipError_t launch_force(int ndim, int precision, int stage, bool energy,
                         const AtomArrays &A, const ListView &Lv, const PairView &P,
                         const Box &box, const StepCoef &c, const ReduceBuf &R,
                         hipStream_t s) {
+    if (energy != (stage == kStageSums)) return hipErrorInvalidValue;
     if (A.n <= 0) return hipSuccess;
+    if (Lv.tiled && Lv.built)
+        return launch_force_tiled(ndim, precision, stage, energy, A, Lv, P, box, c, R, s);
     if (ndim == 3)
         launch_force_d<3>(precision, stage, energy, A, Lv, P, box, c, R, s);
     else

@@ -1,0 +1,397 @@
+// step_tiled.hip -- pair forces over the tiled list (tiles.hpp), fused with
+// the integrator stages exactly as k_force in step.hip.
+//
+// Reference behaviour restated (paths relative to the reference root):
+//   NListed<A,P>::set_forces / energy / pressure / stress
+//                                   src/interaction.hpp:2153-2291
+//   LJAttractRepulsePair::forces/energy  src/interaction.hpp:1270-1298
+//   RepulsionPair::forces/energy         src/interaction.hpp:1538-1556
+//
+// One workgroup per tile, TPA lanes per atom (kWave * TPA threads).  The
+// tile's stencil (its runs, precomputed at the rebuild) is staged in LDS once
+// per launch; each list entry is a staged slot, so a neighbour position is an
+// LDS read instead of a gather from memory, and the list itself is 2 bytes
+// per entry.  Three staging modes:
+//   RAW   (FP64 repulsion): the positions themselves; each pair takes the
+//         reference's minimum image diff(x_i, x_j), so every pair force is
+//         the untiled kernel's bit for bit.
+//   REL64 (FP64 LJ, mixed repulsion): the FP64 minimum-image offset of each
+//         atom from the tile's first atom.  A pair displacement is the
+//         difference of two offsets (both in [-L/2, L/2], so one fold gives
+//         the minimum image; when every offset of the tile is under L/4 the
+//         difference already is, and the fold is skipped).  It differs from
+//         diff(x_i, x_j) only by rounding (~1e-16 relative); the LJ in/out
+//         decision within 1e-14 of the cutoff is taken on the exact d2, so the
+//         set of interacting pairs is the reference's.
+//   REL32 (mixed LJ): the same offsets rounded to FP32 and differenced in
+//         FP32 (half the LDS traffic; the rounding of offsets of a few sigma
+//         costs ~1e-6 sigma in r).  Repulsion does not use it: its force
+//         eps (1 - R/sigma)/sigma cancels at small overlaps.
+// Mixed precision decides in/out within 1e-5 of the cutoff in FP64 on the
+// exact d2, as the untiled mixed path does.
+#include <type_traits>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "pairs.hpp"
+#include "stepcore.hpp"
+#include "tiles.hpp"
+
+namespace parm {
+
+// FP64 LJAttractRepulsePair on a displacement r that may differ from the
+// reference's by rounding; exact_d2() is the reference's d2, used only within
+// 1e-14 of the cutoff (where lim_lo / lim_hi bracket it).
+template <int D, bool ENERGY, typename Exact>
+__device__ __forceinline__ void lj64x(const PairParam &pp, const Vec<D> &r,
+                                      const Exact &exact_d2, double (&f)[D], PairAcc &acc) {
+    const double d2 = sqnorm<D>(r);
+    if (!(d2 < pp.lim_lo)) {
+        if (d2 > pp.lim_hi) return;
+        if (exact_d2() / pp.sig2 > pp.cut2) return;
+    }
+    const double idsq = rcp64(d2);
+    const double ir = pp.sig2 * idsq;
+    const double rsix = ir * ir * ir;
+    const double fmagR = pp.p6 * rsix * (rsix - 1.0);
+    const double fs = fmagR * idsq;
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = fma(fs, r.c[k], f[k]);
+    if (ENERGY) {
+        const double mid = 1.0 - rsix;
+        const double e = pp.eps * (mid * mid) - pp.p5;
+        acc.e += e;
+        acc.w += fmagR;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int b = 0; b < D; b++) acc.s[a * D + b] += r.c[a] * (r.c[b] * fs);
+        acc.contacts += (e != 0.0) ? 1.0 : 0.0;
+        acc.overlaps += (e > 0.0) ? 1.0 : 0.0;
+    }
+}
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, int TPA>
+__global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListView Lv,
+                                                            PairView P, Box box, StepCoef c,
+                                                            ReduceBuf R) {
+    using PT = typename PosStore<D>::T;
+    constexpr bool ENERGY = STAGE == kStageSums;
+    constexpr bool RAW = PREC == kFP64 && KIND != kPairLJ;
+    constexpr bool REL32 = PREC != kFP64 && KIND == kPairLJ;
+    constexpr int NT = kWave * TPA;
+    extern __shared__ double4 st64[];
+    __shared__ int2 runs[kTileRuns];
+    __shared__ int4 hdr;
+    float4 *st32 = reinterpret_cast<float4 *>(st64);
+    int32_t *stj = REL32 ? reinterpret_cast<int32_t *>(st32 + Lv.smax)
+                         : reinterpret_cast<int32_t *>(st64 + Lv.smax);
+
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    if (threadIdx.x < kTileRuns)
+        runs[threadIdx.x] = Lv.truns[(size_t)t * kTileRuns + threadIdx.x];
+    if (threadIdx.x == 0) hdr = Lv.tiles[t];
+    __syncthreads();
+    const int a0 = hdr.x, tcnt = hdr.y;
+    const int S = hdr.z >= 0 ? hdr.w : 0;
+    const PT *pos = static_cast<const PT *>(A.pos_in);
+    // the first wavefront updates the tile's atoms at the end (one lane per
+    // atom); their inputs are loaded now, to arrive during the pair loop
+    // (FP64 loads them at the end instead: its pair loop needs the registers)
+    constexpr bool PREFETCH = PREC != kFP64;
+    AtomIn<D> ain;
+    const bool upd = STAGE != kStageSums && threadIdx.x < (unsigned)tcnt;
+    if (PREFETCH && upd) atom_load<D, STAGE>(A, a0 + (int)threadIdx.x, ain);
+    Vec<D> xo;
+#pragma unroll
+    for (int k = 0; k < D; k++) xo.c[k] = 0.0;
+    if (!RAW && S > 0) xo = PosStore<D>::load(pos[a0]);
+    int wide = 0;  // some offset reaches L/4: pair differences need the fold
+    for (int s = threadIdx.x; s < S; s += NT) {
+        const int r = slot_run(runs, s);
+        const int j = runs[r].x + (s - runs[r].y);
+        const Vec<D> xj = PosStore<D>::load(pos[j]);
+        const int tj = SINGLE ? 0 : A.type[j];
+        Vec<D> v = xj;
+        if (!RAW) {
+            v = diff<D>(xj, xo, box);
+#pragma unroll
+            for (int k = 0; k < D; k++) wide |= fabs(v.c[k]) >= 0.5 * box.halfL[k];
+            stj[s] = j;
+        }
+        if (REL32) {
+            float4 q;
+            q.x = (float)v.c[0];
+            q.y = (float)v.c[1];
+            q.z = D == 3 ? (float)v.c[2] : 0.0f;
+            q.w = __int_as_float(tj);
+            st32[s] = q;
+        } else {
+            double4 q;
+            q.x = v.c[0];
+            q.y = v.c[1];
+            if (D == 3) {
+                q.z = v.c[2];
+                q.w = (double)tj;
+            } else {
+                q.z = (double)tj;
+                q.w = 0.0;
+            }
+            st64[s] = q;
+        }
+    }
+    const bool fold = __syncthreads_or(wide) != 0;
+
+    const int li = threadIdx.x / TPA, sub = threadIdx.x % TPA;
+    const int i = a0 + li;
+    const bool act = li < tcnt;
+    PairAcc acc;
+    double fi[D];
+    float f32[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        fi[k] = 0.0;
+        f32[k] = 0.0f;
+    }
+    Vec<D> xi;
+#pragma unroll
+    for (int k = 0; k < D; k++) xi.c[k] = 0.0;
+    if (act) xi = PosStore<D>::load(pos[i]);
+    const bool member = act && hdr.z >= 0 && A.rank[i] >= 0;
+    if (member) {
+        const int cnt = Lv.ncount[i];
+        const int nlane = cnt > sub ? (cnt - sub + TPA - 1) / TPA : 0;
+        const uint32_t *wp = Lv.nbr + (size_t)i * TPA + sub;
+        const size_t wstride = (size_t)Lv.npad * TPA;
+        const PairParam *prow = P.tab;
+        PairParam p0;
+        if (SINGLE)
+            p0 = P.tab[0];
+        else
+            prow = P.tab + (size_t)A.type[i] * P.ntypes;
+        Vec<D> ri64 = xi;
+        float ri[D], Lf[D], hLf[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) ri[k] = Lf[k] = hLf[k] = 0.0f;
+        if (!RAW) {
+            ri64 = diff<D>(xi, xo, box);
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                ri[k] = (float)ri64.c[k];
+                Lf[k] = (float)box.L[k];
+                hLf[k] = (float)box.halfL[k];
+            }
+        }
+        // the reference's d2 (for in/out decisions at the cutoff)
+        auto exact_d2 = [&](int s) {
+            const Vec<D> r = diff<D>(xi, PosStore<D>::load(pos[stj[s]]), box);
+            return sqnorm<D>(r);
+        };
+        auto pair = [&](int s, auto fold_c) {
+            constexpr bool FOLD = decltype(fold_c)::value;
+            if constexpr (RAW) {
+                const double4 q = st64[s];
+                Vec<D> xj;
+                xj.c[0] = q.x;
+                xj.c[1] = q.y;
+                if constexpr (D == 3) xj.c[2] = q.z;
+                const int tj = SINGLE ? 0 : (int)(D == 3 ? q.w : q.z);
+                const Vec<D> r = diff<D>(xi, xj, box);
+                visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : prow[tj], r, fi, f32, acc);
+            } else if constexpr (REL32) {
+                const float4 q = st32[s];
+                const float qc[3] = {q.x, q.y, q.z};
+                float rf[D];
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    float d = ri[k] - qc[k];
+                    if (FOLD) d = d > hLf[k] ? d - Lf[k] : (d < -hLf[k] ? d + Lf[k] : d);
+                    rf[k] = d;
+                }
+                const int tj = SINGLE ? 0 : __float_as_int(q.w);
+                lj32f<D, ENERGY>(SINGLE ? p0 : prow[tj], rf, [&]() { return exact_d2(s); },
+                                 f32, acc);
+            } else {
+                const double4 q = st64[s];
+                const double qc[3] = {q.x, q.y, q.z};
+                Vec<D> r;
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    double d = ri64.c[k] - qc[k];
+                    if (FOLD)
+                        d = d > box.halfL[k] ? d - box.L[k]
+                                             : (d < -box.halfL[k] ? d + box.L[k] : d);
+                    r.c[k] = d;
+                }
+                const int tj = SINGLE ? 0 : (int)(D == 3 ? q.w : q.z);
+                const PairParam &pp = SINGLE ? p0 : prow[tj];
+                if constexpr (PREC == kFP64) {
+                    lj64x<D, ENERGY>(pp, r, [&]() { return exact_d2(s); }, fi, acc);
+                } else {
+                    float rf[D];
+#pragma unroll
+                    for (int k = 0; k < D; k++) rf[k] = (float)r.c[k];
+                    rep32f<D, ENERGY, HARM>(pp, rf, [&]() { return exact_d2(s); }, f32, acc);
+                }
+            }
+        };
+        // two lane entries per 32-bit word; the next word is loaded while
+        // the current one is evaluated
+        auto loop = [&](auto fold_c) {
+            uint32_t wv = nlane > 0 ? wp[0] : 0u;
+            for (int m = 0; m < nlane; m += 2) {
+                const uint32_t cur = wv;
+                if (m + 2 < nlane) wv = wp[(size_t)((m >> 1) + 1) * wstride];
+                pair((int)(cur & 0xffffu), fold_c);
+                if (m + 1 < nlane) pair((int)(cur >> 16), fold_c);
+            }
+        };
+        if (fold)
+            loop(std::integral_constant<bool, true>());
+        else
+            loop(std::integral_constant<bool, false>());
+    }
+    if (PREC != kFP64) {
+#pragma unroll
+        for (int k = 0; k < D; k++) fi[k] += (double)f32[k];
+    }
+#pragma unroll
+    for (int o = 1; o < TPA; o <<= 1) {
+#pragma unroll
+        for (int k = 0; k < D; k++) fi[k] += __shfl_xor(fi[k], o);
+    }
+    // The integrator's per-atom memory traffic is done by the first
+    // wavefront, one lane per atom (64 consecutive atoms: coalesced), with
+    // the summed forces passed through LDS.
+    Top2 top;
+    double nonfinite = 0.0;
+    if (STAGE != kStageSums) {
+        __shared__ double fsum[D][kTileAtoms];
+        if (act && sub == 0) {
+#pragma unroll
+            for (int k = 0; k < D; k++) fsum[k][li] = fi[k];
+        }
+        __syncthreads();
+        if (upd) {
+            double fa[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) fa[k] = fsum[k][threadIdx.x];
+            if (!PREFETCH) atom_load<D, STAGE>(A, a0 + (int)threadIdx.x, ain);
+            atom_store<D, STAGE>(A, a0 + (int)threadIdx.x, fa, ain, c, top, nonfinite);
+        }
+    }
+    group_partials<D, STAGE, ENERGY, TPA>(acc, top, nonfinite, R);
+}
+
+// ======================================================== host launchers
+namespace {
+
+size_t tiled_force_lds(int prec, int kind, int smax) {
+    if (prec == kFP64 && kind != kPairLJ) return (size_t)smax * sizeof(double4);
+    if (prec != kFP64 && kind == kPairLJ)
+        return (size_t)smax * (sizeof(float4) + sizeof(int32_t));
+    return (size_t)smax * (sizeof(double4) + sizeof(int32_t));
+}
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, int TPA>
+void tiled_tt(const AtomArrays &A, const ListView &Lv, const PairView &P, const Box &box,
+              const StepCoef &c, const ReduceBuf &R, hipStream_t s) {
+    constexpr bool DRIFT =
+        (STAGE == kStageVerletKickDrift || STAGE == kStageDampedKickDrift);
+    constexpr bool ENERGY = STAGE == kStageSums;
+    const size_t lds = tiled_force_lds(PREC, KIND, Lv.smax);
+    hipLaunchKernelGGL((k_force_tiled<D, KIND, PREC, SINGLE, HARM, STAGE, TPA>),
+                       dim3(Lv.ntiles), dim3(kWave * TPA), lds, s, A, Lv, P, box, c, R);
+    if (ENERGY || DRIFT) (void)launch_finalize_rt(ENERGY, DRIFT, Lv.ntiles, R, s);
+}
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE>
+void tiled_t(const AtomArrays &A, const ListView &Lv, const PairView &P, const Box &box,
+             const StepCoef &c, const ReduceBuf &R, hipStream_t s) {
+    switch (Lv.tpa) {
+    case 1:
+        tiled_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 1>(A, Lv, P, box, c, R, s);
+        break;
+    case 2:
+        tiled_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 2>(A, Lv, P, box, c, R, s);
+        break;
+    case 8:
+        tiled_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 8>(A, Lv, P, box, c, R, s);
+        break;
+    default:
+        tiled_tt<D, KIND, PREC, SINGLE, HARM, STAGE, 4>(A, Lv, P, box, c, R, s);
+        break;
+    }
+}
+
+template <int D, int KIND, int PREC, bool SINGLE, bool HARM>
+void tiled_s(int stage, const AtomArrays &A, const ListView &Lv, const PairView &P,
+             const Box &box, const StepCoef &c, const ReduceBuf &R, hipStream_t s) {
+    switch (stage) {
+#define PARM_TSTAGE(S)                                                    \
+    case S:                                                               \
+        tiled_t<D, KIND, PREC, SINGLE, HARM, S>(A, Lv, P, box, c, R, s); \
+        break;
+        PARM_TSTAGE(kStageForceOnly)
+        PARM_TSTAGE(kStageForceA)
+        PARM_TSTAGE(kStageVerletKick)
+        PARM_TSTAGE(kStageVerletKickDrift)
+        PARM_TSTAGE(kStageDampedKick)
+        PARM_TSTAGE(kStageDampedKickDrift)
+        PARM_TSTAGE(kStageSums)
+        PARM_TSTAGE(kStageForceAdd)
+#undef PARM_TSTAGE
+    }
+}
+
+template <int D, int KIND, int PREC>
+void tiled_k(int stage, const AtomArrays &A, const ListView &Lv, const PairView &P,
+             const Box &box, const StepCoef &c, const ReduceBuf &R, hipStream_t s) {
+    if constexpr (KIND == kPairRepulsion) {
+        if (P.harmonic) {
+            if (P.single)
+                tiled_s<D, KIND, PREC, true, true>(stage, A, Lv, P, box, c, R, s);
+            else
+                tiled_s<D, KIND, PREC, false, true>(stage, A, Lv, P, box, c, R, s);
+            return;
+        }
+    }
+    if (P.single)
+        tiled_s<D, KIND, PREC, true, false>(stage, A, Lv, P, box, c, R, s);
+    else
+        tiled_s<D, KIND, PREC, false, false>(stage, A, Lv, P, box, c, R, s);
+}
+
+template <int D>
+void tiled_d(int precision, int stage, const AtomArrays &A, const ListView &Lv,
+             const PairView &P, const Box &box, const StepCoef &c, const ReduceBuf &R,
+             hipStream_t s) {
+    if (P.kind == kPairRepulsion) {
+        if (precision == kFP64)
+            tiled_k<D, kPairRepulsion, kFP64>(stage, A, Lv, P, box, c, R, s);
+        else
+            tiled_k<D, kPairRepulsion, kMixed>(stage, A, Lv, P, box, c, R, s);
+    } else {
+        if (precision == kFP64)
+            tiled_k<D, kPairLJ, kFP64>(stage, A, Lv, P, box, c, R, s);
+        else
+            tiled_k<D, kPairLJ, kMixed>(stage, A, Lv, P, box, c, R, s);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_force_tiled(int ndim, int precision, int stage, bool energy,
+                              const AtomArrays &A, const ListView &Lv, const PairView &P,
+                              const Box &box, const StepCoef &c, const ReduceBuf &R,
+                              hipStream_t s) {
+    if (energy != (stage == kStageSums)) return hipErrorInvalidValue;
+    if (A.n <= 0 || Lv.ntiles <= 0) return hipSuccess;
+    if (ndim == 3)
+        tiled_d<3>(precision, stage, A, Lv, P, box, c, R, s);
+    else
+        tiled_d<2>(precision, stage, A, Lv, P, box, c, R, s);
+    return hipGetLastError();
+}
+
+}  // namespace parm

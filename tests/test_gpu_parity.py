@@ -243,6 +243,53 @@ def test_list_update_order_with_zero_skin(integrator):
     assert dev.which() == 1 + 24  # the initial build + one per step
 
 
+@pytest.mark.parametrize("kind,ndim,n,rho,skin,bi", [
+    (O.LJ_ATTRACT_REPULSE, 3, 30000, 0.8, 0.3, False),   # row tiles, normal stencils
+    (O.LJ_ATTRACT_REPULSE, 3, 4096, 0.8, 0.3, False),    # full-row stencils
+    (O.REPULSION, 3, 20000, 0.65, 0.2, True),            # few atoms per cell
+    (O.REPULSION, 2, 20000, 1.0, 0.2, True),
+])
+@pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED])
+def test_tiled_layout_matches_untiled(kind, ndim, n, rho, skin, bi, prec, monkeypatch):
+    """The LDS-tiled list and force kernels against the untiled ones on the
+    same state: identical pair sets, forces within summation-order rounding
+    (FP64) or the mixed bar, and identical trajectories' rebuild counts."""
+    _device_or_skip()
+    print(f"tiled vs untiled kind={kind} ndim={ndim} n={n} prec={prec}", flush=True)
+    x, L = S.jittered_lattice(n, rho, ndim, seed=n + 5)
+    p = S.bidisperse_repulsion(n, n) if bi else S.lj_params(n)
+    v = S.gaussian_velocities(n, ndim, 1.0, seed=9)
+    sys_ = {}
+    for tiled in (1, 0):
+        monkeypatch.setenv("PARM_TILED", str(tiled))
+        s = _make(kind, x, L, p, skin, prec=prec)
+        s.set_state(v=v)
+        s.initialize()
+        sys_[tiled] = s
+    a, b = sys_[1], sys_[0]
+    fa, la = a.pairs()
+    fb, lb = b.pairs()
+    assert np.array_equal(fa, fb) and np.array_equal(la, lb)
+    ref = O.forces(x, L, _table(kind, p), fb, lb)
+    for s in (a, b):
+        d = np.sqrt(np.sum((s.forces() - ref["f"]) ** 2, axis=1))
+        if prec == capi.FP64:
+            assert np.all(d <= 1e-12 * ref["fabs"] + 1e-15)
+        else:
+            # FP32 pair arithmetic: 2e-5 of sum_j |f_ij|, with a floor of
+            # 1e-6 eps/sigma for atoms whose contacts barely overlap (the
+            # harmonic force eps (1 - R/sigma)/sigma cancels in FP32 there)
+            assert np.all(d <= 2e-5 * ref["fabs"] + 1e-6), np.max(d - 2e-5 * ref["fabs"])
+    ua, ub = a.pair_sums(), b.pair_sums()
+    assert abs(ua["U"] - ref["U"]) <= (1e-12 if prec == capi.FP64 else 1e-5) * abs(ref["U"])
+    assert ua["contacts"] == ub["contacts"]
+    for s in (a, b):
+        s.verlet(0.002, 30)
+    assert a.which() == b.which()
+    xa_, xb_ = a.positions(), b.positions()
+    assert np.abs(xa_ - xb_).max() <= (1e-10 if prec == capi.FP64 else 1e-5) * L[0]
+
+
 def test_mixed_precision_force_error():
     _device_or_skip()
     n = 4096
