@@ -347,7 +347,7 @@ int build_list(parm_system *s, const void *at) {
     if (s->tiled) {
         PARM_HIP(launch_build_tiled(s->ndim, s->n, s->ntiles, s->smax, s->tiles, at, s->xwrap,
                                     s->rank, s->diam, s->keys, s->cstart, s->grid, s->box,
-                                    s->skin, s->tpa, s->cap, s->npad,
+                                    s->skin, s->dmax, s->tpa, s->cap, s->npad,
                                     reinterpret_cast<uint16_t *>(s->nbr), s->ncount, s->xref,
                                     s->bpart, s->stats, s->dflag + kSlotBuild, st));
     } else {

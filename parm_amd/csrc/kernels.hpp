@@ -54,7 +54,8 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const int4 
                               const void *pos, void *xw, const int32_t *rank,
                               const double *diam, const uint32_t *keys,
                               const int32_t *cstart, const CellGrid &g, const Box &box,
-                              double skin, int tpa, int cap, int npad, uint16_t *nb16,
+                              double skin, double dmax, int tpa, int cap, int npad,
+                              uint16_t *nb16,
                               int32_t *ncount, void *xref, unsigned long long *part,
                               unsigned long long *stats, uint32_t *flag, hipStream_t s);
 hipError_t launch_tiled_to_global(int ndim, int ntiles, const int4 *tiles,

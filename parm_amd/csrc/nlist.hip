@@ -528,45 +528,58 @@ __global__ __launch_bounds__(kBlock) void k_tile_sizes(int cap_tiles, CellGrid g
 }
 
 // Tiled list build: one workgroup per tile, G lanes per atom.  The tile's
-// stencil is staged in LDS as wrapped positions shifted by the stencil
-// cell's periodic image (so |w_i - staged| is the minimum-image distance up
-// to rounding) with the list diameter, plus the atom index for the exact
-// test.  Each atom then scans one contiguous staged range per stencil row.
-// The predicate is the one k_build_list evaluates (1e-6 prefilter band, the
-// reference's exact test inside it); entries are staged slots.
+// stencil is staged in LDS as FP32 offsets from the tile's first atom (its
+// wrapped position), taken from the wrapped positions shifted by the stencil
+// cell's periodic image, with the list diameter in the spare lane.  Each atom
+// then scans one contiguous staged range per stencil row (two for an end cell
+// of a full-row stencil).
+//
+// The FP32 test is only a prefilter.  With M the largest offset magnitude in
+// the tile (and L along folded dimensions), every FP32 displacement component
+// is within e = 2^-23 (M + L + c) of the exact one, so |d2_32 - d2| <=
+// 2 sqrt(D) c e + 4 ulp(c^2); the tolerance is four times that.  Candidates
+// farther than the tolerance from the cutoff are decided by the FP32 test;
+// the others by the reference's exact predicate (remainder minimum image,
+// sqrt(d2) < (d_i + d_j)/2 + skin) on the unwrapped FP64 positions, so the
+// pair set is the reference's exactly.  Entries are staged slots.
 template <int D, int G, int U>
 __global__ __launch_bounds__(kBlock) void k_build_tiled(
     const TileRec *__restrict__ tiles, int smax, const typename PosStore<D>::T *__restrict__ pos,
     const typename PosStore<D>::T *__restrict__ xw, const int32_t *__restrict__ rank,
     const double *__restrict__ diam, const uint32_t *__restrict__ keys,
-    const int32_t *__restrict__ cstart, CellGrid g, Box box, double skin, int tpa, int cap,
-    int npad, uint16_t *__restrict__ nb16, int32_t *__restrict__ ncount,
+    const int32_t *__restrict__ cstart, CellGrid g, Box box, double skin, double dmax, int tpa,
+    int cap, int npad, uint16_t *__restrict__ nb16, int32_t *__restrict__ ncount,
     typename PosStore<D>::T *__restrict__ xref, unsigned long long *__restrict__ part,
     uint32_t *__restrict__ flag) {
-    extern __shared__ double4 st_pos[];
+    extern __shared__ float4 st_f[];
     __shared__ TileLayout L;
     __shared__ unsigned long long red[kBlock / kWave][2];
+    __shared__ int mbits;
     const int t = xcd_block(blockIdx.x, gridDim.x);
+    if (threadIdx.x == 0) mbits = 0;
     tile_layout<D>(tiles, t, keys, cstart, g, L);
     const int S = L.off[L.NR * L.W];
+    Vec<D> wo;
+#pragma unroll
+    for (int k = 0; k < D; k++) wo.c[k] = 0.0;
+    if (S > 0) wo = PosStore<D>::load(xw[L.a0]);
+    float mloc = 0.0f;
     for (int s = threadIdx.x; s < S; s += kBlock) {
         const int c = slot_cell(L, s);
         int img[3];
         (void)stencil_cell<D>(g, L.cy, L.cz, L.xa, c / L.W, c % L.W, img);
         const int j = L.g0[c] + (s - L.off[c]);
         const Vec<D> w = PosStore<D>::load(xw[j]);
-        double4 v;
-        v.x = w.c[0] + img[0] * box.L[0];
-        v.y = D >= 2 ? w.c[1] + img[1] * box.L[1] : 0.0;
-        if (D == 3) {
-            v.z = w.c[2] + img[2] * box.L[2];
-            v.w = diam[j];
-        } else {
-            v.z = diam[j];
-            v.w = 0.0;
+        float r[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            r[k] = (float)((w.c[k] + img[k] * box.L[k]) - wo.c[k]);
+            mloc = fmaxf(mloc, fabsf(r[k]));
         }
-        st_pos[s] = v;
+        const float dj = (float)diam[j];
+        st_f[s] = D == 3 ? float4{r[0], r[1], r[2], dj} : float4{r[0], r[1], dj, 0.0f};
     }
+    if (mloc > 0.0f) atomicMax(&mbits, __float_as_int(mloc));  // >= 0: int order
     __syncthreads();
     const int li = threadIdx.x / G, sub = threadIdx.x % G;
     const int lane = threadIdx.x % kWave;
@@ -584,7 +597,28 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
             const Vec<D> xi = PosStore<D>::load(pi);
             const Vec<D> wi = PosStore<D>::load(xw[i]);
             const double di = diam[i];
+            const float dif = (float)di;
             const int cx = (int)(keys[i] % (uint32_t)g.nc[0]);
+            // along x the minimum image is explicit for a collapsed
+            // dimension or a full-row stencil (no staged shift)
+            const bool fold0 = g.nc[0] < 3 || L.xfull;
+            bool foldk[3];
+            float Lf[3], iLf[3], ri[3];
+            double Lfold = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                foldk[k] = k == 0 ? fold0 : g.nc[k] < 3;
+                Lf[k] = (float)box.L[k];
+                iLf[k] = (float)box.invL[k];
+                ri[k] = (float)(wi.c[k] - wo.c[k]);
+                if (foldk[k]) Lfold = fmax(Lfold, box.L[k]);
+            }
+            // prefilter tolerance on d2 (see above)
+            const double cmax = dmax + skin;
+            const double e = 0x1p-23 * ((double)__int_as_float(mbits) + Lfold + cmax);
+            const float tol = (float)(4.0 * (2.0 * sqrt((double)D) * cmax * e +
+                                             4.0 * 0x1p-24 * cmax * cmax));
+            const float skf = (float)skin;
             // atom i's own staged slot (centre stencil row, its cell)
             int self = -1;
             {
@@ -594,9 +628,6 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
                 const int cc = kc * L.W + uc;
                 self = L.off[cc] + (i - L.g0[cc]);
             }
-            // along x the minimum image is explicit for a collapsed
-            // dimension or a full-row stencil (no staged shift)
-            const bool fold0 = g.nc[0] < 3 || L.xfull;
             int cnt = 0;
             for (int kr = 0; kr < 2 * L.NR; kr++) {
                 int lo, hi, lo1, hi1;
@@ -606,38 +637,35 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
                     hi = hi1;
                 }
                 for (int base = lo; base < hi; base += G * U) {
-                    double4 pw[U];
+                    float4 pw[U];
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         const int s = base + u * G + sub;
-                        pw[u] = st_pos[s < hi ? s : lo];
+                        pw[u] = st_f[s < hi ? s : lo];
                     }
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         const int s = base + u * G + sub;
                         bool pass = false;
                         if (s < hi && s != self) {
-                            {
-                                const double pc[3] = {pw[u].x, pw[u].y, pw[u].z};
-                                const double dj = D == 3 ? pw[u].w : pw[u].z;
-                                double d2w = 0.0;
+                            const float pc[3] = {pw[u].x, pw[u].y, pw[u].z};
+                            const float djf = D == 3 ? pw[u].w : pw[u].z;
+                            float d2 = 0.0f;
 #pragma unroll
-                                for (int k2 = 0; k2 < D; k2++) {
-                                    double dk = wi.c[k2] - pc[k2];
-                                    if (k2 == 0 ? fold0 : g.nc[k2] < 3)
-                                        dk -= box.L[k2] * rint(dk * box.invL[k2]);
-                                    d2w = fma(dk, dk, d2w);
-                                }
-                                const double cc = (di + dj) / 2 + skin;
-                                const double c2 = cc * cc;
-                                if (d2w < c2 * (1.0 - 1e-6))
-                                    pass = true;
-                                else if (d2w <= c2 * (1.0 + 1e-6)) {
-                                    const int cj = slot_cell(L, s);
-                                    const int j = L.g0[cj] + (s - L.off[cj]);
-                                    pass = in_list<D>(xi, PosStore<D>::load(pos[j]), di, dj,
-                                                      box, skin);
-                                }
+                            for (int k2 = 0; k2 < D; k2++) {
+                                float dk = ri[k2] - pc[k2];
+                                if (foldk[k2]) dk -= Lf[k2] * rintf(dk * iLf[k2]);
+                                d2 = fmaf(dk, dk, d2);
+                            }
+                            const float cc = fmaf(0.5f, dif + djf, skf);
+                            const float c2 = cc * cc;
+                            if (d2 < c2 - tol) {
+                                pass = true;
+                            } else if (d2 <= c2 + tol) {
+                                const int cj = slot_cell(L, s);
+                                const int j = L.g0[cj] + (s - L.off[cj]);
+                                pass = in_list<D>(xi, PosStore<D>::load(pos[j]), di, diam[j],
+                                                  box, skin);
                             }
                         }
                         const unsigned long long m = (__ballot(pass) >> gbase) & gmask;
@@ -885,13 +913,13 @@ hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys
 constexpr int kTiledLanes = 4;   // lanes per atom in k_build_tiled
 constexpr int kTiledUnroll = 4;
 
-size_t tiled_build_lds(int smax) { return (size_t)smax * sizeof(double4); }
+size_t tiled_build_lds(int smax) { return (size_t)smax * sizeof(float4); }
 
 hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileRec *tiles,
                               const void *pos, void *xw, const int32_t *rank,
                               const double *diam, const uint32_t *keys,
                               const int32_t *cstart, const CellGrid &g, const Box &box,
-                              double skin, int tpa, int cap, int npad, uint16_t *nb16,
+                              double skin, double dmax, int tpa, int cap, int npad, uint16_t *nb16,
                               int32_t *ncount, void *xref, unsigned long long *part,
                               unsigned long long *stats, uint32_t *flag, hipStream_t s) {
     static_assert(kTiledLanes * kTileAtoms == kBlock, "one tile per workgroup");
@@ -902,14 +930,14 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
                            (const double4 *)pos, g, box, (double4 *)xw);
         hipLaunchKernelGGL((k_build_tiled<3, kTiledLanes, kTiledUnroll>), dim3(ntiles),
                            dim3(kBlock), lds, s, tiles, smax, (const double4 *)pos,
-                           (const double4 *)xw, rank, diam, keys, cstart, g, box, skin, tpa,
+                           (const double4 *)xw, rank, diam, keys, cstart, g, box, skin, dmax, tpa,
                            cap, npad, nb16, ncount, (double4 *)xref, part, flag);
     } else {
         hipLaunchKernelGGL(k_wrap<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double2 *)pos, g, box, (double2 *)xw);
         hipLaunchKernelGGL((k_build_tiled<2, kTiledLanes, kTiledUnroll>), dim3(ntiles),
                            dim3(kBlock), lds, s, tiles, smax, (const double2 *)pos,
-                           (const double2 *)xw, rank, diam, keys, cstart, g, box, skin, tpa,
+                           (const double2 *)xw, rank, diam, keys, cstart, g, box, skin, dmax, tpa,
                            cap, npad, nb16, ncount, (double2 *)xref, part, flag);
     }
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, ntiles, part, stats);
