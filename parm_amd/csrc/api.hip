@@ -315,7 +315,7 @@ int plan_tiles(parm_system *s) {
     s->tile_ok = false;
     s->ntiles = 0;
     s->smax = 0;
-    if (s->layout_pref == 2 || s->n == 0) return PARM_OK;
+    if (s->n == 0) return PARM_OK;
     const int nrows = s->grid.ncell / s->grid.nc[0];
     const int need = 2 * (s->n / 64) + nrows + 4;
     if (need > s->tiles_cap) {
@@ -334,7 +334,7 @@ int plan_tiles(parm_system *s) {
     const int pstride = force_blocks(std::max(s->n, 1), 8);
     if (bad || nt <= 0 || nt > pstride || tiled_build_lds(smax) > s->lds_max) return PARM_OK;
     s->tile_ok = true;
-    s->tiled = true;
+    s->tiled = s->layout_pref != 2;
     s->ntiles = nt;
     s->smax = std::max(smax, 1);
     return PARM_OK;
@@ -344,11 +344,12 @@ int plan_tiles(parm_system *s) {
 // relayout) in the layout chosen by plan_tiles.
 int build_list(parm_system *s, const void *at) {
     hipStream_t st = s->stream;
-    if (s->tiled) {
+    if (s->tile_ok) {
+        // the tiled build writes either list layout
         PARM_HIP(launch_build_tiled(s->ndim, s->n, s->ntiles, s->smax, s->tiles, at, s->xwrap,
-                                    s->rank, s->diam, s->keys, s->cstart, s->grid, s->box,
-                                    s->skin, s->dmax, s->tpa, s->cap, s->npad,
-                                    reinterpret_cast<uint16_t *>(s->nbr), s->ncount, s->xref,
+                                    s->rank, s->diam, s->type, s->type_bits, s->keys,
+                                    s->cstart, s->grid, s->box, s->skin, s->dmax, s->tpa,
+                                    s->cap, s->npad, !s->tiled, s->nbr, s->ncount, s->xref,
                                     s->bpart, s->stats, s->dflag + kSlotBuild, st));
     } else {
         PARM_HIP(launch_build_list(s->ndim, s->n, at, s->xwrap, s->rank, s->diam, s->type,

@@ -52,12 +52,12 @@ hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys
 size_t tiled_build_lds(int smax);
 hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const int4 *tiles,
                               const void *pos, void *xw, const int32_t *rank,
-                              const double *diam, const uint32_t *keys,
-                              const int32_t *cstart, const CellGrid &g, const Box &box,
-                              double skin, double dmax, int tpa, int cap, int npad,
-                              uint16_t *nb16,
-                              int32_t *ncount, void *xref, unsigned long long *part,
-                              unsigned long long *stats, uint32_t *flag, hipStream_t s);
+                              const double *diam, const int32_t *type, int type_bits,
+                              const uint32_t *keys, const int32_t *cstart, const CellGrid &g,
+                              const Box &box, double skin, double dmax, int tpa, int cap,
+                              int npad, bool out32, void *nbr, int32_t *ncount, void *xref,
+                              unsigned long long *part, unsigned long long *stats,
+                              uint32_t *flag, hipStream_t s);
 hipError_t launch_tiled_to_global(int ndim, int ntiles, const int4 *tiles,
                                   const uint32_t *keys, const int32_t *cstart,
                                   const CellGrid &g, const uint16_t *nb16,

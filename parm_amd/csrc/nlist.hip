@@ -542,16 +542,20 @@ __global__ __launch_bounds__(kBlock) void k_tile_sizes(int cap_tiles, CellGrid g
 // the others by the reference's exact predicate (remainder minimum image,
 // sqrt(d2) < (d_i + d_j)/2 + skin) on the unwrapped FP64 positions, so the
 // pair set is the reference's exactly.  Entries are staged slots.
-template <int D, int G, int U>
+// OUT32: write untiled entries ((j << type_bits) | type_j, k_build_list's
+// layout) instead of staged slots, for systems whose force kernel runs on the
+// untiled list.
+template <int D, int G, int U, bool OUT32>
 __global__ __launch_bounds__(kBlock) void k_build_tiled(
     const TileRec *__restrict__ tiles, int smax, const typename PosStore<D>::T *__restrict__ pos,
     const typename PosStore<D>::T *__restrict__ xw, const int32_t *__restrict__ rank,
-    const double *__restrict__ diam, const uint32_t *__restrict__ keys,
-    const int32_t *__restrict__ cstart, CellGrid g, Box box, double skin, double dmax, int tpa,
-    int cap, int npad, uint16_t *__restrict__ nb16, int32_t *__restrict__ ncount,
-    typename PosStore<D>::T *__restrict__ xref, unsigned long long *__restrict__ part,
-    uint32_t *__restrict__ flag) {
+    const double *__restrict__ diam, const int32_t *__restrict__ type, int type_bits,
+    const uint32_t *__restrict__ keys, const int32_t *__restrict__ cstart, CellGrid g, Box box,
+    double skin, double dmax, int tpa, int cap, int npad, void *__restrict__ nbr,
+    int32_t *__restrict__ ncount, typename PosStore<D>::T *__restrict__ xref,
+    unsigned long long *__restrict__ part, uint32_t *__restrict__ flag) {
     extern __shared__ float4 st_f[];
+    uint32_t *st_e = reinterpret_cast<uint32_t *>(st_f + smax);  // OUT32 entries
     __shared__ TileLayout L;
     __shared__ unsigned long long red[kBlock / kWave][2];
     __shared__ int mbits;
@@ -578,6 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
         }
         const float dj = (float)diam[j];
         st_f[s] = D == 3 ? float4{r[0], r[1], r[2], dj} : float4{r[0], r[1], dj, 0.0f};
+        if (OUT32) st_e[s] = ((uint32_t)j << type_bits) | (uint32_t)type[j];
     }
     if (mloc > 0.0f) atomicMax(&mbits, __float_as_int(mloc));  // >= 0: int order
     __syncthreads();
@@ -671,7 +676,15 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
                         const unsigned long long m = (__ballot(pass) >> gbase) & gmask;
                         if (pass) {
                             const int q = cnt + __popcll(m & below);
-                            if (q < cap) nb16[tiled_entry(q, i, tpa, npad)] = (uint16_t)s;
+                            if (q < cap) {
+                                if (OUT32)
+                                    static_cast<uint32_t *>(
+                                        nbr)[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] =
+                                        st_e[s];
+                                else
+                                    static_cast<uint16_t *>(nbr)[tiled_entry(q, i, tpa, npad)] =
+                                        (uint16_t)s;
+                            }
                         }
                         cnt += __popcll(m);
                     }
@@ -913,33 +926,40 @@ hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys
 constexpr int kTiledLanes = 4;   // lanes per atom in k_build_tiled
 constexpr int kTiledUnroll = 4;
 
-size_t tiled_build_lds(int smax) { return (size_t)smax * sizeof(float4); }
+size_t tiled_build_lds(int smax) { return (size_t)smax * (sizeof(float4) + sizeof(uint32_t)); }
 
 hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileRec *tiles,
                               const void *pos, void *xw, const int32_t *rank,
-                              const double *diam, const uint32_t *keys,
-                              const int32_t *cstart, const CellGrid &g, const Box &box,
-                              double skin, double dmax, int tpa, int cap, int npad, uint16_t *nb16,
-                              int32_t *ncount, void *xref, unsigned long long *part,
-                              unsigned long long *stats, uint32_t *flag, hipStream_t s) {
+                              const double *diam, const int32_t *type, int type_bits,
+                              const uint32_t *keys, const int32_t *cstart, const CellGrid &g,
+                              const Box &box, double skin, double dmax, int tpa, int cap,
+                              int npad, bool out32, void *nbr, int32_t *ncount, void *xref,
+                              unsigned long long *part, unsigned long long *stats,
+                              uint32_t *flag, hipStream_t s) {
     static_assert(kTiledLanes * kTileAtoms == kBlock, "one tile per workgroup");
     if (n <= 0 || ntiles <= 0) return hipSuccess;
-    const size_t lds = tiled_build_lds(smax);
+    const size_t lds = (size_t)smax * (sizeof(float4) + (out32 ? sizeof(uint32_t) : 0));
+#define PARM_BUILD_TILED(DD, PT, O32)                                                        \
+    hipLaunchKernelGGL((k_build_tiled<DD, kTiledLanes, kTiledUnroll, O32>), dim3(ntiles),      \
+                       dim3(kBlock), lds, s, tiles, smax, (const PT *)pos, (const PT *)xw,     \
+                       rank, diam, type, type_bits, keys, cstart, g, box, skin, dmax, tpa,     \
+                       cap, npad, nbr, ncount, (PT *)xref, part, flag)
     if (ndim == 3) {
         hipLaunchKernelGGL(k_wrap<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double4 *)pos, g, box, (double4 *)xw);
-        hipLaunchKernelGGL((k_build_tiled<3, kTiledLanes, kTiledUnroll>), dim3(ntiles),
-                           dim3(kBlock), lds, s, tiles, smax, (const double4 *)pos,
-                           (const double4 *)xw, rank, diam, keys, cstart, g, box, skin, dmax, tpa,
-                           cap, npad, nb16, ncount, (double4 *)xref, part, flag);
+        if (out32)
+            PARM_BUILD_TILED(3, double4, true);
+        else
+            PARM_BUILD_TILED(3, double4, false);
     } else {
         hipLaunchKernelGGL(k_wrap<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double2 *)pos, g, box, (double2 *)xw);
-        hipLaunchKernelGGL((k_build_tiled<2, kTiledLanes, kTiledUnroll>), dim3(ntiles),
-                           dim3(kBlock), lds, s, tiles, smax, (const double2 *)pos,
-                           (const double2 *)xw, rank, diam, keys, cstart, g, box, skin, dmax, tpa,
-                           cap, npad, nb16, ncount, (double2 *)xref, part, flag);
+        if (out32)
+            PARM_BUILD_TILED(2, double2, true);
+        else
+            PARM_BUILD_TILED(2, double2, false);
     }
+#undef PARM_BUILD_TILED
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, ntiles, part, stats);
     return hipGetLastError();
 }
