@@ -121,6 +121,7 @@ struct parm_system {
     bool tpa_fixed = false, tuned = false;
     uint32_t *nbr = nullptr;       // untiled: uint32 entries; tiled: uint16 staged slots
     int32_t *ncount = nullptr;
+    uint16_t *lcount = nullptr;   // per-lane entry counts, npad x 8
     // tiled layout (tiles.hpp): the current list is tiled when `tiled`
     bool tiled = false, tile_ok = false;
     int layout_pref = 0;  // 0: tiled when possible, until tuned; 1: tiled; 2: untiled
@@ -174,6 +175,7 @@ struct parm_system {
         ListView L{};
         L.nbr = nbr;
         L.ncount = ncount;
+        L.lcount = lcount;
         L.npad = npad;
         L.type_bits = type_bits;
         L.tpa = tpa;
@@ -223,6 +225,7 @@ int set_device(const parm_system *s) {
 void free_list(parm_system *s) {
     dfree(s->nbr);
     dfree(s->ncount);
+    dfree(s->lcount);
     s->cap = 0;
 }
 
@@ -263,6 +266,7 @@ int ensure_list_capacity(parm_system *s, int cap) {
     s->cap = (cap + 7) & ~7;
     PARM_CHECK(dalloc(&s->nbr, (size_t)s->cap * s->npad));
     PARM_CHECK(dalloc(&s->ncount, (size_t)s->npad));
+    PARM_CHECK(dalloc(&s->lcount, (size_t)s->npad * 8));
     return PARM_OK;
 }
 
@@ -349,15 +353,37 @@ int build_list(parm_system *s, const void *at) {
         PARM_HIP(launch_build_tiled(s->ndim, s->n, s->ntiles, s->smax, s->tiles, at, s->xwrap,
                                     s->rank, s->diam, s->type, s->type_bits, s->keys,
                                     s->cstart, s->grid, s->box, s->skin, s->dmax, s->tpa,
-                                    s->cap, s->npad, !s->tiled, s->nbr, s->ncount, s->xref,
+                                    s->cap, s->npad, !s->tiled, s->nbr, s->ncount, s->lcount,
+                                    s->xref,
                                     s->bpart, s->stats, s->dflag + kSlotBuild, st));
     } else {
         PARM_HIP(launch_build_list(s->ndim, s->n, at, s->xwrap, s->rank, s->diam, s->type,
                                    s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
                                    s->type_bits, s->tpa, s->cap, s->npad, s->nbr, s->ncount,
-                                   s->xref, s->bpart, s->stats, s->dflag + kSlotBuild, st));
+                                   s->lcount, s->xref, s->bpart, s->stats, s->dflag + kSlotBuild, st));
     }
     return PARM_OK;
+}
+
+// build_list until the list fits: the capacity a layout needs is the
+// fullest lane's count times the lanes per atom, so it can grow on a rebuild
+// and on a change of lanes per atom alike.  Sets npairs.
+int build_checked(parm_system *s, const void *at) {
+    hipStream_t st = s->stream;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
+        PARM_HIP(hipMemsetAsync(s->dflag + kSlotBuild, 0, sizeof(uint32_t), st));
+        PARM_CHECK(build_list(s, at));
+        unsigned long long hs[2] = {0, 0};
+        PARM_HIP(hipMemcpyAsync(hs, s->stats, sizeof(hs), hipMemcpyDeviceToHost, st));
+        PARM_HIP(hipStreamSynchronize(st));
+        if (hs[1] <= (unsigned long long)s->cap) {
+            s->npairs = (int64_t)(hs[0] / 2);
+            return PARM_OK;
+        }
+        PARM_CHECK(ensure_list_capacity(s, (int)(hs[1] + hs[1] / 8 + 8)));
+    }
+    return fail(PARM_E_DEVICE, "neighbour list build did not converge");
 }
 
 // NeighborList::update_list's rebuild branch (src/trackers.cpp:55-69):
@@ -385,23 +411,11 @@ int rebuild(parm_system *s) {
     PARM_HIP(launch_cell_bounds(n, s->grid.ncell, s->keys, s->cstart, st));
     if (s->cap == 0) PARM_CHECK(ensure_list_capacity(s, initial_capacity(s)));
     PARM_CHECK(plan_tiles(s));
-    for (int attempt = 0; attempt < 4; attempt++) {
-        PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
-        PARM_HIP(hipMemsetAsync(s->dflag + kSlotBuild, 0, sizeof(uint32_t), st));
-        PARM_CHECK(build_list(s, s->pos));
-        unsigned long long hs[2] = {0, 0};
-        PARM_HIP(hipMemcpyAsync(hs, s->stats, sizeof(hs), hipMemcpyDeviceToHost, st));
-        PARM_HIP(hipStreamSynchronize(st));
-        if (hs[1] <= (unsigned long long)s->cap) {
-            s->npairs = (int64_t)(hs[0] / 2);
-            s->built = true;
-            s->which++;
-            s->ignorechanged = false;
-            return PARM_OK;
-        }
-        PARM_CHECK(ensure_list_capacity(s, (int)(hs[1] + hs[1] / 8 + 8)));
-    }
-    return fail(PARM_E_DEVICE, "neighbour list build did not converge");
+    PARM_CHECK(build_checked(s, s->pos));
+    s->built = true;
+    s->which++;
+    s->ignorechanged = false;
+    return PARM_OK;
 }
 
 int read_flag(parm_system *s, int idx, uint32_t *flag) {
@@ -478,8 +492,7 @@ struct ProfScope {
 // with the cell keys and bounds of that rebuild.
 int relayout(parm_system *s) {
     if (!s->built || s->n == 0) return PARM_OK;
-    PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), s->stream));
-    PARM_CHECK(build_list(s, s->xref));
+    PARM_CHECK(build_checked(s, s->xref));
     return PARM_OK;
 }
 
@@ -1033,7 +1046,7 @@ int parm_nlist_pairs(parm_system *s, int64_t cap, int32_t *first, int32_t *last,
     // Export: read the full list back and keep, per member i, the entries of
     // lower member rank, sorted by rank -- the reference's pair order.
     const int n = s->n;
-    std::vector<int32_t> cnt(n), orig(n), rank(n);
+    std::vector<int32_t> orig(n), rank(n);
     std::vector<uint32_t> nb((size_t)s->cap * s->npad);
     const uint32_t *src = s->nbr;
     uint32_t *conv = nullptr;
@@ -1043,15 +1056,16 @@ int parm_nlist_pairs(parm_system *s, int64_t cap, int32_t *first, int32_t *last,
         hipError_t e = launch_tiled_to_global(s->ndim, s->ntiles, s->tiles, s->keys,
                                               s->cstart, s->grid,
                                               reinterpret_cast<const uint16_t *>(s->nbr),
-                                              s->ncount, s->type, s->type_bits, s->tpa,
-                                              s->cap, s->npad, conv, s->stream);
+                                              s->lcount, s->type, s->type_bits, s->tpa,
+                                              s->npad, conv, s->stream);
         if (e != hipSuccess) {
             dfree(conv);
             return fail(PARM_E_DEVICE, std::string("pair export: ") + hipGetErrorString(e));
         }
         src = conv;
     }
-    PARM_HIP(hipMemcpyAsync(cnt.data(), s->ncount, sizeof(int32_t) * n,
+    std::vector<uint16_t> lcnt((size_t)n * s->tpa);
+    PARM_HIP(hipMemcpyAsync(lcnt.data(), s->lcount, sizeof(uint16_t) * lcnt.size(),
                             hipMemcpyDeviceToHost, s->stream));
     PARM_HIP(hipMemcpyAsync(orig.data(), s->orig, sizeof(int32_t) * n,
                             hipMemcpyDeviceToHost, s->stream));
@@ -1071,10 +1085,13 @@ int parm_nlist_pairs(parm_system *s, int64_t cap, int32_t *first, int32_t *last,
     for (int64_t r = 0; r < s->nmem; r++) {
         const int i = slot_of_rank[r];
         lower.clear();
-        for (int q = 0; q < cnt[i]; q++) {
-            const size_t idx = ((size_t)(q / s->tpa) * s->npad + i) * s->tpa + q % s->tpa;
-            const int j = (int)(nb[idx] >> s->type_bits);
-            if (rank[j] < r) lower.push_back(rank[j]);
+        for (int sub = 0; sub < s->tpa; sub++) {
+            const int nl = lcnt[(size_t)i * s->tpa + sub];
+            for (int m = 0; m < nl; m++) {
+                const size_t idx = ((size_t)m * s->npad + i) * s->tpa + sub;
+                const int j = (int)(nb[idx] >> s->type_bits);
+                if (rank[j] < r) lower.push_back(rank[j]);
+            }
         }
         std::sort(lower.begin(), lower.end());
         for (int32_t rj : lower) {

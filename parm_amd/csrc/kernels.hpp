@@ -39,9 +39,9 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                              const uint32_t *keys, const int32_t *cstart,
                              const CellGrid &g, const Box &box, double skin, double dmax,
                              int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
-                             int32_t *ncount, void *xref, unsigned long long *part,
-                             unsigned long long *stats, uint32_t *flag,
-                             hipStream_t s);
+                             int32_t *ncount, uint16_t *lcount, void *xref,
+                             unsigned long long *part, unsigned long long *stats,
+                             uint32_t *flag, hipStream_t s);
 
 // Tiled layout (tiles.hpp).  launch_tiles fills tiles (cap_tiles records)
 // and info[0] = tiles, info[1] = largest staged stencil, info[2] != 0 when the
@@ -55,14 +55,14 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const int4 
                               const double *diam, const int32_t *type, int type_bits,
                               const uint32_t *keys, const int32_t *cstart, const CellGrid &g,
                               const Box &box, double skin, double dmax, int tpa, int cap,
-                              int npad, bool out32, void *nbr, int32_t *ncount, void *xref,
-                              unsigned long long *part, unsigned long long *stats,
-                              uint32_t *flag, hipStream_t s);
+                              int npad, bool out32, void *nbr, int32_t *ncount,
+                              uint16_t *lcount, void *xref, unsigned long long *part,
+                              unsigned long long *stats, uint32_t *flag, hipStream_t s);
 hipError_t launch_tiled_to_global(int ndim, int ntiles, const int4 *tiles,
                                   const uint32_t *keys, const int32_t *cstart,
                                   const CellGrid &g, const uint16_t *nb16,
-                                  const int32_t *ncount, const int32_t *type, int type_bits,
-                                  int tpa, int cap, int npad, uint32_t *out, hipStream_t s);
+                                  const uint16_t *lcount, const int32_t *type, int type_bits,
+                                  int tpa, int npad, uint32_t *out, hipStream_t s);
 
 // -------------------------------------------------------------- step.hip
 enum PairKind : int { kPairNone = 0, kPairLJ = 1, kPairRepulsion = 2 };
@@ -99,6 +99,7 @@ struct AtomArrays {
 struct ListView {
     const uint32_t *nbr;
     const int32_t *ncount;
+    const uint16_t *lcount;  // entries of lane sub of slot i: lcount[i * tpa + sub]
     int npad;
     int type_bits;
     int tpa;               // lanes per atom; entries grouped by tpa (see k_force)

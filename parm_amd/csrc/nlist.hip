@@ -209,7 +209,8 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
     const int32_t *__restrict__ type, const uint32_t *__restrict__ keys,
     const int32_t *__restrict__ cstart, CellGrid g, Box box, double skin, double dmax,
     int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
-    int32_t *__restrict__ ncount, typename PosStore<D>::T *__restrict__ xref,
+    int32_t *__restrict__ ncount, uint16_t *__restrict__ lcount,
+    typename PosStore<D>::T *__restrict__ xref,
     unsigned long long *__restrict__ part /* per workgroup: sum, max */,
     uint32_t *__restrict__ flag) {
     __shared__ unsigned long long red[kBlock / kWave][2];
@@ -224,7 +225,10 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
         const typename PosStore<D>::T pi = pos[i];
         if (sub == 0) xref[i] = pi;
         if (rank[i] < 0) {
-            if (sub == 0) ncount[i] = 0;
+            if (sub == 0) {
+                ncount[i] = 0;
+                for (int l = 0; l < tpa; l++) lcount[(size_t)i * tpa + l] = 0;
+            }
         } else {
             const Vec<D> xi = PosStore<D>::load(pi);
             const Vec<D> wi = PosStore<D>::load(xw[i]);
@@ -358,7 +362,11 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
                 }
             }
             if (sub == 0) {
-                ncount[i] = cnt < cap ? cnt : cap;
+                const int cc = cnt < cap ? cnt : cap;
+                ncount[i] = cc;
+                // entry q is lane q % tpa's entry q / tpa
+                for (int l = 0; l < tpa; l++)
+                    lcount[(size_t)i * tpa + l] = (uint16_t)(cc > l ? (cc - l + tpa - 1) / tpa : 0);
                 if (cnt > cap) atomicOr(flag, kFlagOverflow);
                 mycnt = (unsigned long long)cnt;
             }
@@ -542,22 +550,29 @@ __global__ __launch_bounds__(kBlock) void k_tile_sizes(int cap_tiles, CellGrid g
 // the others by the reference's exact predicate (remainder minimum image,
 // sqrt(d2) < (d_i + d_j)/2 + skin) on the unwrapped FP64 positions, so the
 // pair set is the reference's exactly.  Entries are staged slots.
+// G lanes per atom.  In each round a lane tests U candidates; one prefix
+// scan over the atom's G lanes then places the round's passing candidates in
+// candidate-lane order, so entry q of the atom goes to the force kernel's
+// lane q % tpa (its entry q / tpa) and every lane holds ceil or floor of
+// count / tpa entries.  lcount[i * tpa + l] is lane l's count.
 // OUT32: write untiled entries ((j << type_bits) | type_j, k_build_list's
 // layout) instead of staged slots, for systems whose force kernel runs on the
 // untiled list.
 template <int D, int G, int U, bool OUT32>
-__global__ __launch_bounds__(kBlock) void k_build_tiled(
+__global__ __launch_bounds__(kWave *G) void k_build_tiled(
     const TileRec *__restrict__ tiles, int smax, const typename PosStore<D>::T *__restrict__ pos,
     const typename PosStore<D>::T *__restrict__ xw, const int32_t *__restrict__ rank,
     const double *__restrict__ diam, const int32_t *__restrict__ type, int type_bits,
     const uint32_t *__restrict__ keys, const int32_t *__restrict__ cstart, CellGrid g, Box box,
     double skin, double dmax, int tpa, int cap, int npad, void *__restrict__ nbr,
-    int32_t *__restrict__ ncount, typename PosStore<D>::T *__restrict__ xref,
-    unsigned long long *__restrict__ part, uint32_t *__restrict__ flag) {
+    int32_t *__restrict__ ncount, uint16_t *__restrict__ lcount,
+    typename PosStore<D>::T *__restrict__ xref, unsigned long long *__restrict__ part,
+    uint32_t *__restrict__ flag) {
+    constexpr int NT = kWave * G;
     extern __shared__ float4 st_f[];
     uint32_t *st_e = reinterpret_cast<uint32_t *>(st_f + smax);  // OUT32 entries
     __shared__ TileLayout L;
-    __shared__ unsigned long long red[kBlock / kWave][2];
+    __shared__ unsigned long long red[G][2];
     __shared__ int mbits;
     const int t = xcd_block(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0) mbits = 0;
@@ -568,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
     for (int k = 0; k < D; k++) wo.c[k] = 0.0;
     if (S > 0) wo = PosStore<D>::load(xw[L.a0]);
     float mloc = 0.0f;
-    for (int s = threadIdx.x; s < S; s += kBlock) {
+    for (int s = threadIdx.x; s < S; s += NT) {
         const int c = slot_cell(L, s);
         int img[3];
         (void)stencil_cell<D>(g, L.cy, L.cz, L.xa, c / L.W, c % L.W, img);
@@ -588,16 +603,17 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
     __syncthreads();
     const int li = threadIdx.x / G, sub = threadIdx.x % G;
     const int lane = threadIdx.x % kWave;
-    const int gbase = lane - sub;
-    const unsigned long long below = (sub == 0) ? 0ull : ((1ull << sub) - 1ull);
-    const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
-    unsigned long long mycnt = 0;
+    const int glast = lane - sub + G - 1;  // last lane of this atom's group
+    unsigned long long mysum = 0, myreq = 0;
     if (li < L.cnt) {
         const int i = L.a0 + li;
         const typename PosStore<D>::T pi = pos[i];
         if (sub == 0) xref[i] = pi;
         if (L.row < 0 || rank[i] < 0) {
-            if (sub == 0) ncount[i] = 0;
+            if (sub == 0) {
+                ncount[i] = 0;
+                for (int l = 0; l < tpa; l++) lcount[(size_t)i * tpa + l] = 0;
+            }
         } else {
             const Vec<D> xi = PosStore<D>::load(pi);
             const Vec<D> wi = PosStore<D>::load(xw[i]);
@@ -633,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
                 const int cc = kc * L.W + uc;
                 self = L.off[cc] + (i - L.g0[cc]);
             }
-            int cnt = 0;
+            int cnt = 0;  // the atom's entries so far (same in all its lanes)
             for (int kr = 0; kr < 2 * L.NR; kr++) {
                 int lo, hi, lo1, hi1;
                 stencil_ranges(L, g, kr >> 1, cx, lo, hi, lo1, hi1);
@@ -648,10 +664,12 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
                         const int s = base + u * G + sub;
                         pw[u] = st_f[s < hi ? s : lo];
                     }
+                    bool pass[U];
+                    int np = 0;
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         const int s = base + u * G + sub;
-                        bool pass = false;
+                        pass[u] = false;
                         if (s < hi && s != self) {
                             const float pc[3] = {pw[u].x, pw[u].y, pw[u].z};
                             const float djf = D == 3 ? pw[u].w : pw[u].z;
@@ -665,39 +683,54 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
                             const float cc = fmaf(0.5f, dif + djf, skf);
                             const float c2 = cc * cc;
                             if (d2 < c2 - tol) {
-                                pass = true;
+                                pass[u] = true;
                             } else if (d2 <= c2 + tol) {
                                 const int cj = slot_cell(L, s);
                                 const int j = L.g0[cj] + (s - L.off[cj]);
-                                pass = in_list<D>(xi, PosStore<D>::load(pos[j]), di, diam[j],
-                                                  box, skin);
+                                pass[u] = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
+                                                     diam[j], box, skin);
                             }
                         }
-                        const unsigned long long m = (__ballot(pass) >> gbase) & gmask;
-                        if (pass) {
-                            const int q = cnt + __popcll(m & below);
-                            if (q < cap) {
-                                if (OUT32)
-                                    static_cast<uint32_t *>(
-                                        nbr)[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] =
-                                        st_e[s];
-                                else
-                                    static_cast<uint16_t *>(nbr)[tiled_entry(q, i, tpa, npad)] =
-                                        (uint16_t)s;
-                            }
+                        np += pass[u] ? 1 : 0;
+                    }
+                    // exclusive prefix of the round's passes over the group
+                    int incl = np;
+#pragma unroll
+                    for (int o = 1; o < G; o <<= 1) {
+                        const int y = __shfl_up(incl, o);
+                        if (sub >= o) incl += y;
+                    }
+                    int q = cnt + incl - np;
+                    cnt += __shfl(incl, glast);
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (!pass[u]) continue;
+                        if (q < cap) {
+                            const int s = base + u * G + sub;
+                            if (OUT32)
+                                static_cast<uint32_t *>(
+                                    nbr)[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] = st_e[s];
+                            else
+                                static_cast<uint16_t *>(nbr)[tiled_entry(q, i, tpa, npad)] =
+                                    (uint16_t)s;
                         }
-                        cnt += __popcll(m);
+                        q++;
                     }
                 }
             }
             if (sub == 0) {
-                ncount[i] = cnt < cap ? cnt : cap;
+                const int cc = cnt < cap ? cnt : cap;
+                ncount[i] = cc;
+                for (int l = 0; l < tpa; l++)
+                    lcount[(size_t)i * tpa + l] =
+                        (uint16_t)(cc > l ? (cc - l + tpa - 1) / tpa : 0);
                 if (cnt > cap) atomicOr(flag, kFlagOverflow);
-                mycnt = (unsigned long long)cnt;
+                mysum = (unsigned long long)cnt;
+                myreq = (unsigned long long)cnt;
             }
         }
     }
-    unsigned long long sum = mycnt, mx = mycnt;
+    unsigned long long sum = mysum, mx = myreq;
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
         sum += __shfl_xor(sum, off);
@@ -711,7 +744,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long s2 = 0, m2 = 0;
-        for (int q = 0; q < kBlock / kWave; q++) {
+        for (int q = 0; q < G; q++) {
             s2 += red[q][0];
             m2 = red[q][1] > m2 ? red[q][1] : m2;
         }
@@ -720,27 +753,28 @@ __global__ __launch_bounds__(kBlock) void k_build_tiled(
     }
 }
 
-// Tiled list -> untiled entries ((j << type_bits) | type_j, grouped by tpa),
-// for exporting the pairs.  One workgroup per tile.
+// Tiled list -> untiled entries ((j << type_bits) | type_j at lane sub's
+// entry m: (m * npad + i) * tpa + sub), for exporting the pairs.  One
+// workgroup per tile.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_tiled_to_global(
     const TileRec *__restrict__ tiles, const uint32_t *__restrict__ keys,
     const int32_t *__restrict__ cstart, CellGrid g, const uint16_t *__restrict__ nb16,
-    const int32_t *__restrict__ ncount, const int32_t *__restrict__ type, int type_bits,
-    int tpa, int cap, int npad, uint32_t *__restrict__ out) {
+    const uint16_t *__restrict__ lcount, const int32_t *__restrict__ type, int type_bits,
+    int tpa, int npad, uint32_t *__restrict__ out) {
     __shared__ TileLayout L;
     const int t = blockIdx.x;
     tile_layout<D>(tiles, t, keys, cstart, g, L);
     if (L.row < 0) return;
-    for (int e = threadIdx.x; e < L.cnt * cap; e += kBlock) {
-        const int li = e % L.cnt, q = e / L.cnt;
-        const int i = L.a0 + li;
-        if (q >= ncount[i]) continue;
-        const int s = nb16[tiled_entry(q, i, tpa, npad)];
-        const int c = slot_cell(L, s);
-        const int j = L.g0[c] + (s - L.off[c]);
-        out[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] =
-            ((uint32_t)j << type_bits) | (uint32_t)type[j];
+    for (int e = threadIdx.x; e < L.cnt * tpa; e += kBlock) {
+        const int i = L.a0 + e / tpa, sub = e % tpa;
+        const int nl = lcount[(size_t)i * tpa + sub];
+        for (int m = 0; m < nl; m++) {
+            const int s = nb16[tiled_entry(m * tpa + sub, i, tpa, npad)];
+            const int c = slot_cell(L, s);
+            const int j = L.g0[c] + (s - L.off[c]);
+            out[((size_t)m * npad + i) * tpa + sub] = ((uint32_t)j << type_bits) | (uint32_t)type[j];
+        }
     }
 }
 
@@ -882,9 +916,9 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                              const uint32_t *keys, const int32_t *cstart,
                              const CellGrid &g, const Box &box, double skin, double dmax,
                              int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
-                             int32_t *ncount, void *xref, unsigned long long *part,
-                             unsigned long long *stats, uint32_t *flag,
-                             hipStream_t s) {
+                             int32_t *ncount, uint16_t *lcount, void *xref,
+                             unsigned long long *part, unsigned long long *stats,
+                             uint32_t *flag, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int nb = build_blocks(n);
     if (ndim == 3) {
@@ -893,14 +927,14 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
         hipLaunchKernelGGL((k_build_list<3, kBuildLanes, kBuildUnroll>), dim3(nb), dim3(kBlock), 0, s,
                            n, (const double4 *)pos, (const double4 *)xw, rank, diam, type,
                            keys, cstart, g, box, skin, dmax, type_bits, tpa, cap, npad,
-                           nbr, ncount, (double4 *)xref, part, flag);
+                           nbr, ncount, lcount, (double4 *)xref, part, flag);
     } else {
         hipLaunchKernelGGL(k_wrap<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double2 *)pos, g, box, (double2 *)xw);
         hipLaunchKernelGGL((k_build_list<2, kBuildLanes, kBuildUnroll>), dim3(nb), dim3(kBlock), 0, s,
                            n, (const double2 *)pos, (const double2 *)xw, rank, diam, type,
                            keys, cstart, g, box, skin, dmax, type_bits, tpa, cap, npad,
-                           nbr, ncount, (double2 *)xref, part, flag);
+                           nbr, ncount, lcount, (double2 *)xref, part, flag);
     }
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, nb, part, stats);
     return hipGetLastError();
@@ -933,33 +967,36 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
                               const double *diam, const int32_t *type, int type_bits,
                               const uint32_t *keys, const int32_t *cstart, const CellGrid &g,
                               const Box &box, double skin, double dmax, int tpa, int cap,
-                              int npad, bool out32, void *nbr, int32_t *ncount, void *xref,
-                              unsigned long long *part, unsigned long long *stats,
-                              uint32_t *flag, hipStream_t s) {
-    static_assert(kTiledLanes * kTileAtoms == kBlock, "one tile per workgroup");
+                              int npad, bool out32, void *nbr, int32_t *ncount,
+                              uint16_t *lcount, void *xref, unsigned long long *part,
+                              unsigned long long *stats, uint32_t *flag, hipStream_t s) {
     if (n <= 0 || ntiles <= 0) return hipSuccess;
     const size_t lds = (size_t)smax * (sizeof(float4) + (out32 ? sizeof(uint32_t) : 0));
-#define PARM_BUILD_TILED(DD, PT, O32)                                                        \
-    hipLaunchKernelGGL((k_build_tiled<DD, kTiledLanes, kTiledUnroll, O32>), dim3(ntiles),      \
-                       dim3(kBlock), lds, s, tiles, smax, (const PT *)pos, (const PT *)xw,     \
+#define PARM_BT(DD, PT, GG, O32)                                                              \
+    hipLaunchKernelGGL((k_build_tiled<DD, GG, kTiledUnroll, O32>), dim3(ntiles),               \
+                       dim3(kWave * GG), lds, s, tiles, smax, (const PT *)pos, (const PT *)xw, \
                        rank, diam, type, type_bits, keys, cstart, g, box, skin, dmax, tpa,     \
-                       cap, npad, nbr, ncount, (PT *)xref, part, flag)
+                       cap, npad, nbr, ncount, lcount, (PT *)xref, part, flag)
+#define PARM_BT_G(DD, PT, O32) PARM_BT(DD, PT, kTiledLanes, O32);
     if (ndim == 3) {
         hipLaunchKernelGGL(k_wrap<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double4 *)pos, g, box, (double4 *)xw);
-        if (out32)
-            PARM_BUILD_TILED(3, double4, true);
-        else
-            PARM_BUILD_TILED(3, double4, false);
+        if (out32) {
+            PARM_BT_G(3, double4, true)
+        } else {
+            PARM_BT_G(3, double4, false)
+        }
     } else {
         hipLaunchKernelGGL(k_wrap<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double2 *)pos, g, box, (double2 *)xw);
-        if (out32)
-            PARM_BUILD_TILED(2, double2, true);
-        else
-            PARM_BUILD_TILED(2, double2, false);
+        if (out32) {
+            PARM_BT_G(2, double2, true)
+        } else {
+            PARM_BT_G(2, double2, false)
+        }
     }
-#undef PARM_BUILD_TILED
+#undef PARM_BT_G
+#undef PARM_BT
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, ntiles, part, stats);
     return hipGetLastError();
 }
@@ -967,17 +1004,15 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
 hipError_t launch_tiled_to_global(int ndim, int ntiles, const TileRec *tiles,
                                   const uint32_t *keys, const int32_t *cstart,
                                   const CellGrid &g, const uint16_t *nb16,
-                                  const int32_t *ncount, const int32_t *type, int type_bits,
-                                  int tpa, int cap, int npad, uint32_t *out, hipStream_t s) {
+                                  const uint16_t *lcount, const int32_t *type, int type_bits,
+                                  int tpa, int npad, uint32_t *out, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
     if (ndim == 3)
         hipLaunchKernelGGL(k_tiled_to_global<3>, dim3(ntiles), dim3(kBlock), 0, s, tiles,
-                           keys, cstart, g, nb16, ncount, type, type_bits, tpa, cap, npad,
-                           out);
+                           keys, cstart, g, nb16, lcount, type, type_bits, tpa, npad, out);
     else
         hipLaunchKernelGGL(k_tiled_to_global<2>, dim3(ntiles), dim3(kBlock), 0, s, tiles,
-                           keys, cstart, g, nb16, ncount, type, type_bits, tpa, cap, npad,
-                           out);
+                           keys, cstart, g, nb16, lcount, type, type_bits, tpa, npad, out);
     return hipGetLastError();
 }
 

@@ -152,7 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
     if (act) xi = PosStore<D>::load(pos[i]);
     const bool member = act && A.rank[i] >= 0;
     if (member && Lv.built) {
-        const int cnt = Lv.ncount[i];
+        // lane sub's entries m = 0 .. nl-1 sit at (m * npad + i) * TPA + sub
+        const int nl = Lv.lcount[(size_t)i * TPA + sub];
         const size_t stride = (size_t)Lv.npad * TPA;
         const uint32_t *nb = Lv.nbr + (size_t)i * TPA + sub;
         const int tb = Lv.type_bits;
@@ -163,9 +164,9 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
             p0 = P.tab[0];
         else
             row = P.tab + (size_t)A.type[i] * P.ntypes;
-        int q = sub;
+        int m = 0;
         size_t off = 0;
-        for (; q + TPA < cnt; q += 2 * TPA, off += 2 * stride) {
+        for (; m + 1 < nl; m += 2, off += 2 * stride) {
             const uint32_t e0 = nb[off];
             const uint32_t e1 = nb[off + stride];
             const PT pj0 = pos[e0 >> tb];
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
             visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e1 & tmask], r1, fi,
                                                f32, acc);
         }
-        if (q < cnt) {
+        if (m < nl) {
             const uint32_t e0 = nb[off];
             const Vec<D> r0 = diff<D>(xi, PosStore<D>::load(pos[e0 >> tb]), box);
             visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e0 & tmask], r0, fi,

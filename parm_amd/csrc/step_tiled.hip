@@ -159,8 +159,7 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     if (act) xi = PosStore<D>::load(pos[i]);
     const bool member = act && hdr.z >= 0 && A.rank[i] >= 0;
     if (member) {
-        const int cnt = Lv.ncount[i];
-        const int nlane = cnt > sub ? (cnt - sub + TPA - 1) / TPA : 0;
+        const int nlane = Lv.lcount[(size_t)i * TPA + sub];
         const uint32_t *wp = Lv.nbr + (size_t)i * TPA + sub;
         const size_t wstride = (size_t)Lv.npad * TPA;
         const PairParam *prow = P.tab;
