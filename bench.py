@@ -49,6 +49,16 @@ from parm_amd import capi, workloads  # noqa: E402
 METRIC = "atom-timesteps/s, 1M-atom LJ FP64 vs mixed FP32 at stated drift; packings/s"
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per atom moved by one launch of the FP64 fused step kernel on the
+# 1,048,576-atom LJ workload, by list layout: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes (KiB per dispatch), FETCH_SIZE scaled by 2.0 and
+# WRITE_SIZE by 1.0, the factors that make the same counters read 1 GiB for a
+# 1 GiB device copy (profiles/k_force_lj_1M_pmc.txt).
+TRAFFIC_BYTES_PER_ATOM = {  # (tiled list, lanes per atom) -> bytes per atom
+    (True, 1): 451.3, (True, 2): 450.6, (True, 4): 454.1, (True, 8): 464.8,
+    (False, 1): 603.8, (False, 2): 613.8, (False, 4): 615.4, (False, 8): 619.9,
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -269,6 +279,7 @@ def main():
                                      drift_every=args.drift_every)
     np_pairs = s.numpairs()
     nbar = 2.0 * np_pairs / n
+    layout = s.list_info()
     ms_per_step = 1e3 * wall / args.steps
     value = n * args.steps / wall
     e = np.asarray(energies)
@@ -278,8 +289,13 @@ def main():
     achieved = n * B / (k_avg_ms * 1e-3) / 1e9
     steps_per_rebuild = (args.steps / prof["rebuilds"]) if prof["rebuilds"] else float(
         args.steps)
+    key = (layout["tiled"], layout["lanes"])
+    traffic = (TRAFFIC_BYTES_PER_ATOM[key] * n
+               if n == 1 << 20 and args.skin == 0.3 and key in TRAFFIC_BYTES_PER_ATOM
+               else None)
     log(f"bench: FP64 {value:.3e} atom-steps/s, {ms_per_step:.4f} ms/step, kernel "
-        f"{k_avg_ms * 1e3:.1f} us, {prof['rebuilds']} rebuilds "
+        f"{k_avg_ms * 1e3:.1f} us ({'tiled' if layout['tiled'] else 'untiled'} list, "
+        f"{layout['lanes']} lanes/atom), {prof['rebuilds']} rebuilds "
         f"({prof['rebuild_ms'] / max(prof['rebuilds'], 1):.3f} ms each), nbar {nbar:.1f}, "
         f"drift {drift64}")
     xs_end = None
@@ -306,8 +322,11 @@ def main():
                    "steps_per_rebuild": steps_per_rebuild,
                    "energy_drift_rel_max": drift64, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_force (fused force + kick + drift + skin test)",
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": ("k_force_tiled" if layout["tiled"] else "k_force") +
+                               " (fused force + kick + drift + skin test)",
+                     "list_layout": "tiled" if layout["tiled"] else "untiled",
+                     "lanes_per_atom": layout["lanes"],
                      "kernel_avg_us": k_avg_ms * 1e3,
                      "bytes_per_atom_step": B},
     }
@@ -320,6 +339,7 @@ def main():
         em = np.asarray(en_m)
         drift_m = float(np.max(np.abs(em - em[0])) / abs(em[0]))
         value_m = n * args.steps / wall_m
+        lay_m = sm.list_info()
         sm.close()
         # force error vs FP64 at the initial positions
         a = make_lj(x, v, L, capi.FP64, args.skin)
@@ -331,6 +351,8 @@ def main():
         fn = np.sqrt(np.sum(fa * fa, axis=1))
         out["mixed"] = {
             "value": value_m, "ms_per_step": 1e3 * wall_m / args.steps,
+            "list_layout": "tiled" if lay_m["tiled"] else "untiled",
+            "lanes_per_atom": lay_m["lanes"],
             "speedup_vs_fp64": value_m / value,
             "energy_drift_rel_max": drift_m,
             "force_err_abs_max": float(d.max()), "force_err_abs_rms": float(np.sqrt(np.mean(d * d))),

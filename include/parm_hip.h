@@ -194,6 +194,11 @@ int parm_profile_enable(parm_system *sys, int enable);
 int parm_profile_read(parm_system *sys, int reset, double *step_ms,
                       int64_t *step_launches, double *rebuild_ms,
                       int64_t *rebuilds);
+/* No reference counterpart (diagnostics).  The neighbour-list layout the step
+ * kernels currently use: tiled = 1 for LDS-staged row tiles with 16-bit
+ * entries, 0 for 32-bit atom-index entries; lanes = lanes per atom in the
+ * force kernel; capacity = list entries per atom.  NULL skips. */
+int parm_list_info(parm_system *sys, int *tiled, int *lanes, int *capacity);
 
 /* ---------------------------------------------------------------- batches */
 /* A parm_batch is nsys independent systems, each what the reference builds

@@ -31,7 +31,7 @@ EXPORTS = (
     "parm_set_forces", "parm_add_pair_forces", "parm_pair_sums", "parm_kinetic_energy", "parm_momentum",
     "parm_force_stats", "parm_initialize", "parm_verlet_steps",
     "parm_damped_steps", "parm_synchronize", "parm_profile_enable",
-    "parm_profile_read",
+    "parm_profile_read", "parm_list_info",
     "parm_batch_create", "parm_batch_destroy", "parm_batch_info",
     "parm_batch_set_masses", "parm_batch_set_state", "parm_batch_get_state",
     "parm_batch_pair_repulsion", "parm_batch_pair_lj_attract_repulse",
@@ -101,6 +101,8 @@ def lib():
             "parm_profile_enable": (i32, [_vp, i32]),
             "parm_profile_read": (i32, [_vp, i32, _dp, C.POINTER(C.c_int64), _dp,
                                         C.POINTER(C.c_int64)]),
+            "parm_list_info": (i32, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
             "parm_batch_create": (i32, [i32, i32, i64, C.POINTER(C.c_int64), _dp, dbl,
                                         C.POINTER(_vp)]),
             "parm_batch_destroy": (i32, [_vp]),
@@ -301,6 +303,11 @@ class System:
                                        C.byref(rm), C.byref(rn)))
         return dict(step_ms=sm.value, step_launches=sn.value, rebuild_ms=rm.value,
                     rebuilds=rn.value)
+
+    def list_info(self):
+        t, l, c = C.c_int(0), C.c_int(0), C.c_int(0)
+        _check(lib().parm_list_info(self.h, C.byref(t), C.byref(l), C.byref(c)))
+        return dict(tiled=bool(t.value), lanes=l.value, capacity=c.value)
 
 
 class Batch:

@@ -1272,6 +1272,14 @@ int parm_profile_read(parm_system *s, int reset, double *step_ms, int64_t *step_
     return PARM_OK;
 }
 
+int parm_list_info(parm_system *s, int *tiled, int *lanes, int *capacity) {
+    PARM_CHECK(check_sys(s));
+    if (tiled) *tiled = s->tiled ? 1 : 0;
+    if (lanes) *lanes = s->tpa;
+    if (capacity) *capacity = s->cap;
+    return PARM_OK;
+}
+
 int parm_synchronize(parm_system *s) {
     PARM_CHECK(check_sys(s));
     PARM_CHECK(set_device(s));

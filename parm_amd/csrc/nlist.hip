@@ -797,12 +797,15 @@ __global__ __launch_bounds__(kBlock) void k_wrap(int n,
     xw[i] = PosStore<D>::make(x);
 }
 
-__global__ __launch_bounds__(kBlock) void k_build_stats(
+constexpr int kStatsThreads = 1024;
+
+__global__ __launch_bounds__(kStatsThreads) void k_build_stats(
     int nparts, const unsigned long long *__restrict__ part,
     unsigned long long *__restrict__ stats) {
-    __shared__ unsigned long long red[kBlock / kWave][2];
+    __shared__ unsigned long long red[kStatsThreads / kWave][2];
     unsigned long long sum = 0, mx = 0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock) {
+#pragma unroll 4
+    for (int b = threadIdx.x; b < nparts; b += kStatsThreads) {
         sum += part[2 * (size_t)b];
         mx = part[2 * (size_t)b + 1] > mx ? part[2 * (size_t)b + 1] : mx;
     }
@@ -820,7 +823,7 @@ __global__ __launch_bounds__(kBlock) void k_build_stats(
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long s2 = 0, m2 = 0;
-        for (int q = 0; q < kBlock / kWave; q++) {
+        for (int q = 0; q < kStatsThreads / kWave; q++) {
             s2 += red[q][0];
             m2 = red[q][1] > m2 ? red[q][1] : m2;
         }
@@ -936,7 +939,7 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                            keys, cstart, g, box, skin, dmax, type_bits, tpa, cap, npad,
                            nbr, ncount, lcount, (double2 *)xref, part, flag);
     }
-    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, nb, part, stats);
+    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, nb, part, stats);
     return hipGetLastError();
 }
 
@@ -997,7 +1000,7 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
     }
 #undef PARM_BT_G
 #undef PARM_BT
-    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, s, ntiles, part, stats);
+    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, ntiles, part, stats);
     return hipGetLastError();
 }
 
