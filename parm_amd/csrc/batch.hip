@@ -248,16 +248,36 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
         atomicAdd(&hist[key], 1);
     }
     __syncthreads();
-    // exclusive scan of the counts (thread 0; at most kBatchMaxCells cells)
-    if (t == 0) {
-        int run = 0;
-        for (int c = 0; c < ncell; c++) {
-            const int h = hist[c];
-            cstart[G.cell_off + c] = off + run;
-            hist[c] = run;
-            run += h;
+    // exclusive scan of the counts: each thread sums a run of kScanPer
+    // cells, a block scan of the run sums, then each thread writes its run
+    {
+        constexpr int kScanPer = kBatchMaxCells / kBlock;
+        __shared__ int wsum[kBlock / kWave];
+        const int c0 = t * kScanPer;
+        int loc = 0;
+        for (int q = 0; q < kScanPer; q++) loc += (c0 + q < ncell) ? hist[c0 + q] : 0;
+        const int lane = t % kWave, w = t / kWave;
+        int incl = loc;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
         }
-        cstart[G.cell_off + ncell] = off + run;
+        if (lane == kWave - 1) wsum[w] = incl;
+        __syncthreads();
+        int run = incl - loc;
+        for (int q = 0; q < w; q++) run += wsum[q];
+        __syncthreads();
+        for (int q = 0; q < kScanPer; q++) {
+            const int c = c0 + q;
+            if (c < ncell) {
+                const int h = hist[c];
+                cstart[G.cell_off + c] = off + run;
+                hist[c] = run;
+                run += h;
+            }
+        }
+        if (t == 0) cstart[G.cell_off + ncell] = off + n;
     }
     __syncthreads();
     // stable scatter, one chunk of kBlock atoms at a time
@@ -309,8 +329,24 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
                     for (int u = s0; u < s1; u++) {
                         const int j = cellidx[u];
                         if (j == i) continue;
-                        if (in_list<D>(xi, PosStore<D>::load(pos[j]), di, A.diam[j], G.box,
-                                       skin)) {
+                        // minimum image by rint: equal to the reference's
+                        // remainder up to rounding far inside the 1e-6 band,
+                        // where the reference's predicate decides
+                        const Vec<D> xj = PosStore<D>::load(pos[j]);
+                        const double dj = A.diam[j];
+                        double d2 = 0.0;
+#pragma unroll
+                        for (int k = 0; k < D; k++) {
+                            double dk = xi.c[k] - xj.c[k];
+                            dk -= G.box.L[k] * rint(dk * G.box.invL[k]);
+                            d2 = fma(dk, dk, d2);
+                        }
+                        const double cc = (di + dj) / 2 + skin;
+                        const double c2 = cc * cc;
+                        const bool pass =
+                            d2 < c2 * (1.0 - 1e-6) ||
+                            (d2 <= c2 * (1.0 + 1e-6) && in_list<D>(xi, xj, di, dj, G.box, skin));
+                        if (pass) {
                             if (cnt < cap)
                                 nbr[((size_t)(cnt / tpa) * npad + i) * tpa + cnt % tpa] =
                                     ((uint32_t)j << type_bits) | (uint32_t)A.type[j];
