@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0
 # 1 GiB device copy (profiles/k_force_lj_1M_pmc.txt).
 TRAFFIC_BYTES_PER_ATOM = {  # (tiled list, lanes per atom) -> bytes per atom
     (True, 1): 451.3, (True, 2): 450.6, (True, 4): 454.1, (True, 8): 464.8,
-    (False, 1): 603.8, (False, 2): 613.8, (False, 4): 615.4, (False, 8): 619.9,
+    (False, 1): 603.8, (False, 2): 614.0, (False, 4): 615.4, (False, 8): 619.9,
 }
 
 
