@@ -235,6 +235,130 @@ def batched(args):
     }
 
 
+def relax2d_cpu_baseline(x, types, L, skin, dt, damping, gpu_steps_per_rebuild, steps):
+    """Oracle (single core) on the relaxation workload: a few non-rebuild damped
+    steps at full N on the given configuration, plus the O(N^2) rebuild timed
+    at N = 16384 and scaled by N^2, amortised by the GPU run's rebuild rate."""
+    from oracle import oracle as O
+
+    n = len(x)
+    sig = np.array([1.0, 1.4])
+    t = O.PairTable(O.REPULSION, [1.0, 1.0], sig, [2.0, 2.0], types)
+    sysc = O.System(np.ones(n), L, t, sig[types], skin, ndim=2, brute=False)
+    sysc.set_state(x=x, v=np.zeros_like(x), a=np.zeros_like(x))
+    sysc.initialize()
+    w0 = sysc.which()
+    nsteps = 5
+    t0 = time.perf_counter()
+    sysc.damped(dt, damping, nsteps)
+    t_step = (time.perf_counter() - t0) / nsteps
+    assert sysc.which() == w0, "sample steps must not rebuild"
+    del sysc
+    nr = 16384
+    scale = (nr / n) ** 0.5
+    ts = O.PairTable(O.REPULSION, [1.0, 1.0], sig, [2.0, 2.0], types[:nr])
+    small = O.System(np.ones(nr), L * scale, ts, sig[types[:nr]], skin, ndim=2, brute=True)
+    small.set_state(x=x[:nr] * scale)
+    t0 = time.perf_counter()
+    small.update_list(True)
+    t_reb_small = time.perf_counter() - t0
+    t_reb = t_reb_small * (n / nr) ** 2
+    per_step = t_step + t_reb / max(gpu_steps_per_rebuild, 1.0)
+    return {
+        "value": n / per_step,
+        "unit": "atom-timesteps/s",
+        "cores": 1,
+        "kind": "port",
+        "relaxations_per_s": 1.0 / (per_step * steps),
+        "sample": (f"oracle restatement of ParM's CPU path, 1 core: {nsteps} non-rebuild "
+                   f"damped steps at N={n} on the relaxed configuration ({t_step:.3f} s/step); "
+                   f"O(N^2) rebuild measured at N={nr} ({t_reb_small:.3f} s) and extrapolated "
+                   f"by N^2 to {t_reb:.0f} s, amortised over the GPU run's "
+                   f"{gpu_steps_per_rebuild:.1f} steps per rebuild; relaxations/s = 1 / "
+                   f"({steps} GPU steps x s/step)"),
+    }
+
+
+def relax2d(args):
+    """2D bidisperse soft-disk packing relaxation: N disks, 50/50 diameters 1.0
+    and 1.4, harmonic RepulsionPair (eps = 1, exponent 2), square box at
+    packing fraction 0.84, seeded uniform positions at rest, CollectionDamped
+    dynamics until the mean |F| per disk falls below the threshold (checked
+    every chunk of steps) or the step cap.  Timed from the first damped step
+    to the converged check; the chunk checks are inside the timed region."""
+    prec = capi.MIXED if args.relax_precision == "mixed" else capi.FP64
+    n = args.relax_natoms
+    x, types, L = workloads.bidisperse_packing(n, 0.84, args.seed + 2, 2)
+    sig = np.array([1.0, 1.4])
+
+    def fresh():
+        s = capi.System(2, np.ones(n), prec)
+        s.set_box(L)
+        s.nlist_set(args.relax_skin, sig[types])
+        s.set_repulsion([1.0, 1.0], sig, [2.0, 2.0], types)
+        s.set_state(x=x, v=np.zeros_like(x), a=np.zeros_like(x))
+        s.initialize()
+        s.synchronize()
+        return s
+
+    # untimed warm-up relaxation segment: autotune, first rebuild allocations
+    s = fresh()
+    s.damped(args.relax_dt, args.relax_damping, args.relax_chunk)
+    s.synchronize()
+    s.close()
+
+    s = fresh()
+    w0 = s.which()
+    trace = []
+    steps = 0
+    fmean = s.force_stats()[1]
+    t0 = time.perf_counter()
+    while steps < args.relax_max_steps and fmean >= args.relax_ftol:
+        s.damped(args.relax_dt, args.relax_damping, args.relax_chunk)
+        steps += args.relax_chunk
+        fmax, fmean = s.force_stats()
+        trace.append((steps, fmean))
+    wall = time.perf_counter() - t0
+    for st, fm in trace:
+        log(f"bench: relax2d step {st}: mean |F| {fm:.3e}")
+    rebuilds = s.which() - w0
+    sums = s.pair_sums()
+    xr = s.positions()
+    s.close()
+    converged = bool(fmean < args.relax_ftol)
+    out = {
+        "metric": "atom_timesteps_per_s",
+        "value": n * steps / wall,
+        "unit": "atom-timesteps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "dtype": "f64 state, f32 pair forces" if args.relax_precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D bidisperse harmonic disk packing relaxation (RepulsionPair "
+                               "exponent 2, eps 1, diameters 1.0/1.4 50/50), damped dynamics "
+                               "from uniform random positions to mean |F| < ftol",
+                   "natoms": n, "packing_fraction": 0.84, "box": float(L[0]),
+                   "dt": args.relax_dt, "damping": args.relax_damping,
+                   "skin": args.relax_skin, "ftol": args.relax_ftol,
+                   "check_every": args.relax_chunk, "seed": args.seed + 2},
+        "converged": converged,
+        "wall_s": wall,
+        "relaxations_per_s": (1.0 / wall) if converged else None,
+        "steps_per_rebuild": steps / max(rebuilds, 1),
+        "mean_force_end": fmean,
+        "energy_end": sums["U"],
+        "overlaps_end": sums["overlaps"],
+    }
+    if not args.no_cpu:
+        log("bench: relax2d cpu baseline ...")
+        out["cpu_baseline"] = relax2d_cpu_baseline(
+            xr, types, L, args.relax_skin, args.relax_dt, args.relax_damping,
+            steps / max(rebuilds, 1), steps)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +379,16 @@ def main():
     ap.add_argument("--batch-skin", type=float, default=0.2)
     ap.add_argument("--batch-precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--batch-seq-sample", type=int, default=32)
+    ap.add_argument("--workload", choices=["lj", "relax2d"], default="lj",
+                    help="relax2d: the 2D packing relaxation alone (its own JSON line)")
+    ap.add_argument("--relax-natoms", type=int, default=262144)
+    ap.add_argument("--relax-precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--relax-dt", type=float, default=0.3)
+    ap.add_argument("--relax-damping", type=float, default=0.02)
+    ap.add_argument("--relax-skin", type=float, default=0.4)
+    ap.add_argument("--relax-ftol", type=float, default=1e-10)
+    ap.add_argument("--relax-chunk", type=int, default=250)
+    ap.add_argument("--relax-max-steps", type=int, default=100000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -265,6 +399,9 @@ def main():
     if capi.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "relax2d":
+        print(json.dumps(relax2d(args)), flush=True)
+        return
 
     n = args.natoms
     log(f"bench: building LJ liquid N={n}")

@@ -71,6 +71,20 @@ __device__ __forceinline__ void lj64x(const PairParam &pp, const Vec<D> &r,
     }
 }
 
+// Minimum-image offset of x from the origin o, x - o - L rint((x - o) / L):
+// the staged offsets need not round like the reference's remainder (they are
+// differenced again and carry ~1e-16 relative error anyway).
+template <int D>
+__device__ __forceinline__ Vec<D> offset_from(const Vec<D> &x, const Vec<D> &o, const Box &b) {
+    Vec<D> r;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const double d = x.c[k] - o.c[k];
+        r.c[k] = fma(-b.L[k], rint(d * b.invL[k]), d);
+    }
+    return r;
+}
+
 template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, int TPA>
 __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListView Lv,
                                                             PairView P, Box box, StepCoef c,
@@ -107,14 +121,15 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     for (int k = 0; k < D; k++) xo.c[k] = 0.0;
     if (!RAW && S > 0) xo = PosStore<D>::load(pos[a0]);
     int wide = 0;  // some offset reaches L/4: pair differences need the fold
+    int r = 0;     // run of slot s (slots rise along the loop)
     for (int s = threadIdx.x; s < S; s += NT) {
-        const int r = slot_run(runs, s);
+        while (r + 1 < kTileRuns && runs[r + 1].y <= s) r++;
         const int j = runs[r].x + (s - runs[r].y);
         const Vec<D> xj = PosStore<D>::load(pos[j]);
         const int tj = SINGLE ? 0 : A.type[j];
         Vec<D> v = xj;
         if (!RAW) {
-            v = diff<D>(xj, xo, box);
+            v = offset_from<D>(xj, xo, box);
 #pragma unroll
             for (int k = 0; k < D; k++) wide |= fabs(v.c[k]) >= 0.5 * box.halfL[k];
             stj[s] = j;
@@ -173,7 +188,7 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
 #pragma unroll
         for (int k = 0; k < D; k++) ri[k] = Lf[k] = hLf[k] = 0.0f;
         if (!RAW) {
-            ri64 = diff<D>(xi, xo, box);
+            ri64 = offset_from<D>(xi, xo, box);
 #pragma unroll
             for (int k = 0; k < D; k++) {
                 ri[k] = (float)ri64.c[k];
