@@ -144,7 +144,8 @@ struct parm_system {
     // reductions and flags
     double *part = nullptr, *out = nullptr;
     uint32_t *dflag = nullptr;      // flag words (see kSlot*)
-    uint32_t *hflag = nullptr;      // pinned mirror
+    uint32_t *hflag = nullptr;      // pinned mirror, written by the kernels
+    uint32_t *hflag_dev = nullptr;  // its device-side address
     double *hout = nullptr;         // pinned mirror of out
     hipEvent_t ev_flag = nullptr;
     hipEvent_t ev_ring[kFlagRing] = {};
@@ -206,6 +207,7 @@ struct parm_system {
         R.out = out;
         R.flag = dflag + flag_index;
         R.skin = skin;
+        R.hflag = hflag_dev + flag_index;
         return R;
     }
 };
@@ -418,11 +420,11 @@ int rebuild(parm_system *s) {
     return PARM_OK;
 }
 
+// Flag slot idx once the stream has finished: the skin-test launches store
+// it in the pinned mirror themselves (nothing ran for an empty system).
 int read_flag(parm_system *s, int idx, uint32_t *flag) {
-    PARM_HIP(hipMemcpyAsync(s->hflag + idx, s->dflag + idx, sizeof(uint32_t),
-                            hipMemcpyDeviceToHost, s->stream));
     PARM_HIP(hipStreamSynchronize(s->stream));
-    *flag = s->hflag[idx];
+    *flag = s->n > 0 ? *(volatile uint32_t *)(s->hflag + idx) : 0u;
     return PARM_OK;
 }
 
@@ -430,7 +432,6 @@ int update_list(parm_system *s, bool force, int *rebuilt) {
     if (rebuilt) *rebuilt = 0;
     if (!s->list_cfg) return fail(PARM_E_STATE, "neighbour list not configured");
     if (!force && !s->ignorechanged) {
-        PARM_HIP(hipMemsetAsync(s->dflag + kSlotSkin, 0, sizeof(uint32_t), s->stream));
         PARM_HIP(launch_skin(s->ndim, s->arrays(), s->redbuf(kSlotSkin), s->stream));
         uint32_t fl = 0;
         PARM_CHECK(read_flag(s, kSlotSkin, &fl));
@@ -584,10 +585,11 @@ int autotune(parm_system *s, int stage, const StepCoef &c) {
     return PARM_OK;
 }
 
-// Copies flag slot q to pinned memory and records its ring event.
+// Records the ring event of flag slot q: the launch just queued stores the
+// slot in the pinned mirror (an empty system launches nothing: the slot is
+// cleared here).
 int post_flag(parm_system *s, int slot) {
-    PARM_HIP(hipMemcpyAsync(s->hflag + slot, s->dflag + slot, sizeof(uint32_t),
-                            hipMemcpyDeviceToHost, s->stream));
+    if (s->n == 0) s->hflag[slot] = 0;
     PARM_HIP(hipEventRecord(s->ev_ring[slot], s->stream));
     return PARM_OK;
 }
@@ -611,14 +613,12 @@ int run_steps(parm_system *s, bool damped, const StepCoef &c, int64_t nsteps) {
     if (nsteps >= 16) PARM_CHECK(autotune(s, kick_drift, c));
     hipStream_t st = s->stream;
     // first half of step 0
-    PARM_HIP(hipMemsetAsync(s->dflag, 0, sizeof(uint32_t), st));
     PARM_HIP(launch_drift(s->ndim, kick, s->arrays(), c, s->redbuf(0), st));
     PARM_CHECK(post_flag(s, 0));
     for (int64_t q = 0; q < nsteps; q++) {
         const bool last = (q == nsteps - 1);
         const int slot = (int)(q % kFlagRing), next = (int)((q + 1) % kFlagRing);
         if (!last) {
-            PARM_HIP(hipMemsetAsync(s->dflag + next, 0, sizeof(uint32_t), st));
             {
                 ProfScope ps(s, &s->prof_step);
                 PARM_HIP(launch_force(s->ndim, s->prec, kick_drift, false,
@@ -633,7 +633,7 @@ int run_steps(parm_system *s, bool damped, const StepCoef &c, int64_t nsteps) {
         }
         // update_trackers at the end of step q: skin test of x_{q+1}
         PARM_HIP(hipEventSynchronize(s->ev_ring[slot]));
-        const uint32_t fl = s->hflag[slot];
+        const uint32_t fl = *(volatile uint32_t *)(s->hflag + slot);
         if (fl & kFlagNonFinite)
             return fail(PARM_E_NONFINITE,
                         "position became non-finite at step " + std::to_string(q));
@@ -643,7 +643,6 @@ int run_steps(parm_system *s, bool damped, const StepCoef &c, int64_t nsteps) {
                 PARM_CHECK(rebuild(s));
             }
             if (!last) {
-                PARM_HIP(hipMemsetAsync(s->dflag + next, 0, sizeof(uint32_t), st));
                 AtomArrays A = s->arrays();
                 A.pos_in = s->pos_alt;
                 PARM_HIP(launch_skin(s->ndim, A, s->redbuf(next), st));
@@ -887,7 +886,9 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
         (rc = dalloc(&s->out, kRedWidth)) ||
         (rc = dalloc(&s->dflag, 2 * kFlagRing + 4)) || (rc = dalloc(&s->tinfo, 4)))
         return bail(rc);
-    if (hipHostMalloc((void **)&s->hflag, (2 * kFlagRing + 4) * sizeof(uint32_t), 0) != hipSuccess ||
+    if (hipHostMalloc((void **)&s->hflag, (2 * kFlagRing + 4) * sizeof(uint32_t),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&s->hflag_dev, s->hflag, 0) != hipSuccess ||
         hipHostMalloc((void **)&s->hout, kRedWidth * sizeof(double), 0) != hipSuccess ||
         hipHostMalloc((void **)&s->htinfo, 4 * sizeof(int), 0) != hipSuccess)
         return bail(fail(PARM_E_NOMEM, "pinned host allocation failed"));
@@ -906,6 +907,7 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
     (void)hipMemsetAsync(s->type, 0, n * sizeof(int32_t), st);
     (void)hipMemsetAsync(s->diam, 0, n * sizeof(double), st);
     (void)hipMemsetAsync(s->dflag, 0, (2 * kFlagRing + 4) * sizeof(uint32_t), st);
+    for (int k = 0; k < 2 * kFlagRing + 4; k++) s->hflag[k] = 0;
     hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, st, (int)n, s->orig);
     hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, (int)n, s->rank, -1);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)

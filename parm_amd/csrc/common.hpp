@@ -216,7 +216,8 @@ __device__ __forceinline__ Top2 wave_top2(Top2 t) {
     return t;
 }
 
-// Flag word written by kernels and copied to pinned host memory.
+// Flag word stored by the reduction kernels in device memory and in a pinned
+// host mirror.
 enum : uint32_t {
     kFlagRebuild = 1u,      // skin criterion met: rebuild before the next forces
     kFlagNonFinite = 2u,    // a displacement was NaN/Inf

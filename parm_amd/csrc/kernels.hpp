@@ -147,6 +147,7 @@ struct ReduceBuf {
     double *out;                  // kRedWidth doubles (final sums)
     uint32_t *flag;               // flag word
     double skin;                  // skin test threshold
+    uint32_t *hflag;              // device-visible pinned host copy of the flag word, or null
 };
 constexpr int kRedWidth = 16;     // U, W, S[9], contacts, overlaps, top1, top2, nonfinite
 

@@ -104,14 +104,10 @@ __global__ __launch_bounds__(kFinal) void k_finalize(int nparts, ReduceBuf R) {
             R.out[kTop1] = tt.a;
             R.out[kTop2] = tt.b;
             R.out[kNonFinite] = nft;
-            uint32_t fl = 0;
-            if (nft > 0) fl |= kFlagNonFinite;
-            // NeighborList::update_list (src/trackers.cpp:23-53) checks
-            // bigdist + biggestdist >= skin only after one of the two grew,
-            // which needs a displacement > 0: when nothing moved it never
-            // rebuilds.
-            if (tt.a > 0 && tt.b + tt.a >= R.skin) fl |= kFlagRebuild;
-            *R.flag |= fl;
+            const uint32_t fl = skin_flag(tt, nft, R.skin);
+            *R.flag = fl;
+            if (R.hflag)
+                __hip_atomic_store(R.hflag, fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }

@@ -20,10 +20,23 @@ namespace parm {
 // A kernel that reduces (sums of energy / virial / stress / counts, and the
 // skin test's top-2 and non-finite count) writes one partial per workgroup
 // with plain stores; k_finalize, launched right after it on the same stream,
-// reduces the partials in a fixed order.  No fences or contended atomics are
+// reduces the partials in a fixed order.  (A last-workgroup reduction inside
+// the launch would need a device-scope release per workgroup, which on a
+// multi-XCD part writes back the XCD's L2: far dearer than one more launch.)  No fences or contended atomics are
 // needed inside the hot kernel, and the result does not depend on scheduling.
 constexpr int kSumU = 0, kSumW = 1, kSumS = 2, kSumContacts = 11,
               kSumOverlaps = 12, kTop1 = 13, kTop2 = 14, kNonFinite = 15;
+
+// Flag word of a finished skin test: top-2 displacements tt, nft non-finite
+// displacements.  NeighborList::update_list (src/trackers.cpp:23-53) checks
+// bigdist + biggestdist >= skin only after one of the two grew, which needs a
+// displacement > 0: when nothing moved it never rebuilds.
+__device__ __forceinline__ uint32_t skin_flag(const Top2 &tt, double nft, double skin) {
+    uint32_t fl = 0;
+    if (nft > 0) fl |= kFlagNonFinite;
+    if (tt.a > 0 && tt.b + tt.a >= skin) fl |= kFlagRebuild;
+    return fl;
+}
 
 template <bool SUMS, bool SKIN, int NW>
 __device__ __forceinline__ void block_partial(double (&v)[kRedWidth], Top2 t,
