@@ -100,9 +100,10 @@ struct parm_system {
     double *f[3] = {nullptr, nullptr, nullptr};
     double *m = nullptr, *diam = nullptr;
     int32_t *type = nullptr, *orig = nullptr, *rank = nullptr;
-    // permutation scratch
-    double *spare_d = nullptr;
-    int32_t *spare_i = nullptr;
+    // permutation targets: one spare per permuted column (PermuteSet order)
+    void *pos_spare = nullptr;
+    double *spare_d[kPermD] = {};
+    int32_t *spare_i[kPermI] = {};
     // staging for uploads/downloads (n x ndim doubles, original order)
     double *stage = nullptr;
     double *stage_small = nullptr;  // per-atom scalars / int32 in original order
@@ -130,6 +131,8 @@ struct parm_system {
     int tiles_cap = 0, ntiles = 0, smax = 0;
     size_t lds_max = 0;  // dynamic LDS a tiled workgroup may take
     int *tinfo = nullptr, *htinfo = nullptr;
+    int spec_smax = 0;  // last plan's largest stencil: the next rebuild's speculative bound
+    BuildReport *hrep = nullptr, *hrep_dev = nullptr;  // pinned build report
     uint32_t *keys = nullptr, *tkeys = nullptr, *hist = nullptr;
     int32_t *vals = nullptr, *tvals = nullptr, *cstart = nullptr;
     int cstart_cap = 0;
@@ -284,43 +287,49 @@ int initial_capacity(const parm_system *s) {
 }
 
 // Permute every per-slot array by perm (new slot i <- old slot perm[i]).
+// Every per-slot array into the new slot order, in one launch: the position
+// buffers (the alternate one holds the next step's drifted positions inside
+// a fused step loop; xref is free to take one of them, the list build
+// rewrites it), the ndim components of v, a, f, the masses and diameters,
+// and the type / original index / member rank.  Each lands in its spare,
+// which then becomes the column.
 int permute_all(parm_system *s, const int32_t *perm) {
-    hipStream_t st = s->stream;
-    const int n = s->n;
-    // positions: both buffers (the alternate one holds the next step's
-    // drifted positions inside a fused step loop); xref is the scratch, it is
-    // rewritten by the list build
-    for (void **pb : {&s->pos, &s->pos_alt}) {
-        if (s->ndim == 3) {
-            PARM_HIP(gather<double4>(n, perm, (const double4 *)*pb, (double4 *)s->xref, st));
-        } else {
-            PARM_HIP(gather<double2>(n, perm, (const double2 *)*pb, (double2 *)s->xref, st));
-        }
-        std::swap(*pb, s->xref);
+    PermuteSet P{};
+    P.pos_src[0] = s->pos;
+    P.pos_dst[0] = s->xref;
+    P.pos_src[1] = s->pos_alt;
+    P.pos_dst[1] = s->pos_spare;
+    double **dcols[kPermD];
+    int nd = 0;
+    for (double **vec : {s->v, s->a, s->f})
+        for (int k = 0; k < s->ndim; k++) dcols[nd++] = &vec[k];
+    dcols[nd++] = &s->m;
+    dcols[nd++] = &s->diam;
+    for (int c = 0; c < nd; c++) {
+        P.d_src[c] = *dcols[c];
+        P.d_dst[c] = s->spare_d[c];
     }
-    double **dcols[] = {&s->v[0], &s->v[1], &s->v[2], &s->a[0], &s->a[1], &s->a[2],
-                        &s->f[0], &s->f[1], &s->f[2], &s->m,    &s->diam};
-    for (double **c : dcols) {
-        if (!*c) continue;
-        PARM_HIP(gather<double>(n, perm, *c, s->spare_d, st));
-        std::swap(*c, s->spare_d);
+    P.nd = nd;
+    int32_t **icols[kPermI] = {&s->type, &s->orig, &s->rank};
+    for (int c = 0; c < kPermI; c++) {
+        P.i_src[c] = *icols[c];
+        P.i_dst[c] = s->spare_i[c];
     }
-    int32_t **icols[] = {&s->type, &s->orig, &s->rank};
-    for (int32_t **c : icols) {
-        PARM_HIP(gather<int32_t>(n, perm, *c, s->spare_i, st));
-        std::swap(*c, s->spare_i);
-    }
+    P.ni = kPermI;
+    PARM_HIP(launch_permute(s->ndim, s->n, perm, P, s->stream));
+    std::swap(s->pos, s->xref);
+    std::swap(s->pos_alt, s->pos_spare);
+    for (int c = 0; c < nd; c++) std::swap(*dcols[c], s->spare_d[c]);
+    for (int c = 0; c < kPermI; c++) std::swap(*icols[c], s->spare_i[c]);
     return PARM_OK;
 }
 
-// Tiles of the current cell order (tiles.hpp).  The tiled layout is used
-// when every tile's stencil fits in LDS and the tile count fits the reduction
+// Tiles of the current cell order (tiles.hpp): launch_tile_plan queues the
+// plan (tile records, stencil runs, info = {tiles, largest stencil, bad});
+// accept_tiles decides the layout from it.  The tiled layout is used when
+// every tile's stencil fits in LDS and the tile count fits the reduction
 // scratch; otherwise (or with PARM_TILED=0) the untiled layout.
-int plan_tiles(parm_system *s) {
-    s->tiled = false;
-    s->tile_ok = false;
-    s->ntiles = 0;
-    s->smax = 0;
+int launch_tile_plan(parm_system *s) {
     if (s->n == 0) return PARM_OK;
     const int nrows = s->grid.ncell / s->grid.nc[0];
     const int need = 2 * (s->n / 64) + nrows + 4;
@@ -333,21 +342,38 @@ int plan_tiles(parm_system *s) {
     }
     PARM_HIP(launch_tiles(s->ndim, s->n, s->grid, s->keys, s->cstart, s->tiles_cap, s->tiles,
                           s->truns, s->tinfo, s->stream));
-    PARM_HIP(hipMemcpyAsync(s->htinfo, s->tinfo, 3 * sizeof(int), hipMemcpyDeviceToHost,
-                            s->stream));
-    PARM_HIP(hipStreamSynchronize(s->stream));
-    const int nt = s->htinfo[0], smax = s->htinfo[1], bad = s->htinfo[2];
+    return PARM_OK;
+}
+
+void accept_tiles(parm_system *s, int nt, int smax, int bad) {
+    s->tiled = false;
+    s->tile_ok = false;
+    s->ntiles = 0;
+    s->smax = 0;
     const int pstride = force_blocks(std::max(s->n, 1), 8);
-    if (bad || nt <= 0 || nt > pstride || tiled_build_lds(smax) > s->lds_max) return PARM_OK;
+    if (s->n == 0 || bad || nt <= 0 || nt > pstride || tiled_build_lds(smax) > s->lds_max)
+        return;
     s->tile_ok = true;
     s->tiled = s->layout_pref != 2;
     s->ntiles = nt;
     s->smax = std::max(smax, 1);
+}
+
+// The plan queued by launch_tile_plan, read back (one host round trip).
+int plan_tiles(parm_system *s) {
+    if (s->n == 0) {
+        accept_tiles(s, 0, 0, 0);
+        return PARM_OK;
+    }
+    PARM_HIP(hipMemcpyAsync(s->htinfo, s->tinfo, 3 * sizeof(int), hipMemcpyDeviceToHost,
+                            s->stream));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    accept_tiles(s, s->htinfo[0], s->htinfo[1], s->htinfo[2]);
     return PARM_OK;
 }
 
 // Builds the list at positions `at` (the current ones, or xref for a
-// relayout) in the layout chosen by plan_tiles.
+// relayout) in the layout chosen by accept_tiles.
 int build_list(parm_system *s, const void *at) {
     hipStream_t st = s->stream;
     if (s->tile_ok) {
@@ -356,14 +382,28 @@ int build_list(parm_system *s, const void *at) {
                                     s->rank, s->diam, s->type, s->type_bits, s->keys,
                                     s->cstart, s->grid, s->box, s->skin, s->dmax, s->tpa,
                                     s->cap, s->npad, !s->tiled, s->nbr, s->ncount, s->lcount,
-                                    s->xref,
-                                    s->bpart, s->stats, s->dflag + kSlotBuild, st));
+                                    s->xref, s->bpart, s->stats, s->dflag + kSlotBuild,
+                                    nullptr, s->hrep_dev, st));
     } else {
         PARM_HIP(launch_build_list(s->ndim, s->n, at, s->xwrap, s->rank, s->diam, s->type,
                                    s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
                                    s->type_bits, s->tpa, s->cap, s->npad, s->nbr, s->ncount,
-                                   s->lcount, s->xref, s->bpart, s->stats, s->dflag + kSlotBuild, st));
+                                   s->lcount, s->xref, s->bpart, s->stats, s->dflag + kSlotBuild,
+                                   s->hrep_dev, st));
     }
+    return PARM_OK;
+}
+
+// The report of the build just queued, once the stream has finished.
+int read_report(parm_system *s, BuildReport *r) {
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    const volatile BuildReport *h = s->hrep;
+    r->sum = h->sum;
+    r->max = h->max;
+    r->ntiles = h->ntiles;
+    r->smax = h->smax;
+    r->bad = h->bad;
+    r->flag = h->flag;
     return PARM_OK;
 }
 
@@ -371,21 +411,52 @@ int build_list(parm_system *s, const void *at) {
 // fullest lane's count times the lanes per atom, so it can grow on a rebuild
 // and on a change of lanes per atom alike.  Sets npairs.
 int build_checked(parm_system *s, const void *at) {
-    hipStream_t st = s->stream;
     for (int attempt = 0; attempt < 4; attempt++) {
-        PARM_HIP(hipMemsetAsync(s->stats, 0, 2 * sizeof(unsigned long long), st));
-        PARM_HIP(hipMemsetAsync(s->dflag + kSlotBuild, 0, sizeof(uint32_t), st));
         PARM_CHECK(build_list(s, at));
-        unsigned long long hs[2] = {0, 0};
-        PARM_HIP(hipMemcpyAsync(hs, s->stats, sizeof(hs), hipMemcpyDeviceToHost, st));
-        PARM_HIP(hipStreamSynchronize(st));
-        if (hs[1] <= (unsigned long long)s->cap) {
-            s->npairs = (int64_t)(hs[0] / 2);
+        BuildReport r{};
+        if (s->n == 0) {
+            s->npairs = 0;
             return PARM_OK;
         }
-        PARM_CHECK(ensure_list_capacity(s, (int)(hs[1] + hs[1] / 8 + 8)));
+        PARM_CHECK(read_report(s, &r));
+        if (r.max <= (unsigned long long)s->cap) {
+            s->npairs = (int64_t)(r.sum / 2);
+            return PARM_OK;
+        }
+        PARM_CHECK(ensure_list_capacity(s, (int)(r.max + r.max / 8 + 8)));
     }
     return fail(PARM_E_DEVICE, "neighbour list build did not converge");
+}
+
+// Rebuild-time list build with one host round trip: the tiled build is
+// queued behind the tile plan before the host has read it, with the
+// previous plan's largest stencil (plus a margin) as its LDS bound and the
+// tile capacity as its grid.  The report then carries the plan: when the
+// plan fits the bounds (and the list its capacity) the build stands;
+// otherwise the plan is accepted as read and the list built again.
+int build_speculative(parm_system *s) {
+    const int pstride = force_blocks(std::max(s->n, 1), 8);
+    const int grid = std::min(s->tiles_cap, pstride);
+    const int limit = (int)(s->lds_max / tiled_build_lds(1));
+    const int bound = std::min(limit, s->spec_smax + std::max(s->spec_smax / 8, 64));
+    const bool tiled = s->layout_pref != 2;
+    PARM_HIP(launch_build_tiled(s->ndim, s->n, grid, bound, s->tiles, s->pos, s->xwrap, s->rank,
+                                s->diam, s->type, s->type_bits, s->keys, s->cstart, s->grid,
+                                s->box, s->skin, s->dmax, s->tpa, s->cap, s->npad, !tiled,
+                                s->nbr, s->ncount, s->lcount, s->xref, s->bpart, s->stats,
+                                s->dflag + kSlotBuild, s->tinfo, s->hrep_dev, s->stream));
+    BuildReport r{};
+    PARM_CHECK(read_report(s, &r));
+    accept_tiles(s, r.ntiles, r.smax, r.bad);
+    s->spec_smax = s->tile_ok ? s->smax : 0;
+    const bool fits = s->tile_ok && s->tiled == tiled && r.ntiles <= grid &&
+                      !(r.flag & kFlagReplan);
+    if (fits && r.max <= (unsigned long long)s->cap) {
+        s->npairs = (int64_t)(r.sum / 2);
+        return PARM_OK;
+    }
+    if (fits) PARM_CHECK(ensure_list_capacity(s, (int)(r.max + r.max / 8 + 8)));
+    return build_checked(s, s->pos);
 }
 
 // NeighborList::update_list's rebuild branch (src/trackers.cpp:55-69):
@@ -412,8 +483,14 @@ int rebuild(parm_system *s) {
     }
     PARM_HIP(launch_cell_bounds(n, s->grid.ncell, s->keys, s->cstart, st));
     if (s->cap == 0) PARM_CHECK(ensure_list_capacity(s, initial_capacity(s)));
-    PARM_CHECK(plan_tiles(s));
-    PARM_CHECK(build_checked(s, s->pos));
+    PARM_CHECK(launch_tile_plan(s));
+    if (n > 0 && s->spec_smax > 0) {
+        PARM_CHECK(build_speculative(s));
+    } else {
+        PARM_CHECK(plan_tiles(s));
+        PARM_CHECK(build_checked(s, s->pos));
+        s->spec_smax = s->tile_ok ? s->smax : 0;
+    }
     s->built = true;
     s->which++;
     s->ignorechanged = false;
@@ -709,8 +786,9 @@ void destroy(parm_system *s) {
     dfree(s->type);
     dfree(s->orig);
     dfree(s->rank);
-    dfree(s->spare_d);
-    dfree(s->spare_i);
+    dfree(s->pos_spare);
+    for (double *&p : s->spare_d) dfree(p);
+    for (int32_t *&p : s->spare_i) dfree(p);
     dfree(s->stage);
     dfree(s->stage_small);
     free_list(s);
@@ -731,6 +809,7 @@ void destroy(parm_system *s) {
     dfree(s->out);
     dfree(s->dflag);
     if (s->hflag) (void)hipHostFree(s->hflag);
+    if (s->hrep) (void)hipHostFree(s->hrep);
     if (s->hout) (void)hipHostFree(s->hout);
     if (s->ev_flag) (void)hipEventDestroy(s->ev_flag);
     for (hipEvent_t &e : s->ev_ring)
@@ -865,8 +944,13 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
     if ((rc = dalloc((char **)&s->pos, n * s->pos_bytes)) ||
         (rc = dalloc((char **)&s->pos_alt, n * s->pos_bytes)) ||
         (rc = dalloc((char **)&s->xref, n * s->pos_bytes)) ||
-        (rc = dalloc((char **)&s->xwrap, n * s->pos_bytes)))
+        (rc = dalloc((char **)&s->xwrap, n * s->pos_bytes)) ||
+        (rc = dalloc((char **)&s->pos_spare, n * s->pos_bytes)))
         return bail(rc);
+    for (int c = 0; c < 3 * ndim + 2; c++)
+        if ((rc = dalloc(&s->spare_d[c], n))) return bail(rc);
+    for (int c = 0; c < kPermI; c++)
+        if ((rc = dalloc(&s->spare_i[c], n))) return bail(rc);
     for (int k = 0; k < 3; k++) {
         if ((rc = dalloc(&s->v[k], n)) || (rc = dalloc(&s->a[k], n)) ||
             (rc = dalloc(&s->f[k], n)))
@@ -874,8 +958,7 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
     }
     if ((rc = dalloc(&s->m, n)) || (rc = dalloc(&s->diam, n)) ||
         (rc = dalloc(&s->type, n)) || (rc = dalloc(&s->orig, n)) ||
-        (rc = dalloc(&s->rank, n)) || (rc = dalloc(&s->spare_d, n)) ||
-        (rc = dalloc(&s->spare_i, n)) || (rc = dalloc(&s->stage, n * ndim)) ||
+        (rc = dalloc(&s->rank, n)) || (rc = dalloc(&s->stage, n * ndim)) ||
         (rc = dalloc(&s->stage_small, n)) || (rc = dalloc(&s->keys, n)) ||
         (rc = dalloc(&s->tkeys, n)) || (rc = dalloc(&s->vals, n)) ||
         (rc = dalloc(&s->tvals, n)) ||
@@ -889,6 +972,9 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
     if (hipHostMalloc((void **)&s->hflag, (2 * kFlagRing + 4) * sizeof(uint32_t),
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void **)&s->hflag_dev, s->hflag, 0) != hipSuccess ||
+        hipHostMalloc((void **)&s->hrep, sizeof(BuildReport),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&s->hrep_dev, s->hrep, 0) != hipSuccess ||
         hipHostMalloc((void **)&s->hout, kRedWidth * sizeof(double), 0) != hipSuccess ||
         hipHostMalloc((void **)&s->htinfo, 4 * sizeof(int), 0) != hipSuccess)
         return bail(fail(PARM_E_NOMEM, "pinned host allocation failed"));

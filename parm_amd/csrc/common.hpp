@@ -222,6 +222,7 @@ enum : uint32_t {
     kFlagRebuild = 1u,      // skin criterion met: rebuild before the next forces
     kFlagNonFinite = 2u,    // a displacement was NaN/Inf
     kFlagOverflow = 4u,     // neighbour capacity exceeded during a build
+    kFlagReplan = 8u,       // a speculative tiled build met a tile its bounds do not fit
 };
 
 }  // namespace parm

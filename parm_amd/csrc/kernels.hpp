@@ -24,15 +24,40 @@ int radix_blocks(int n);
 hipError_t launch_cell_keys(int ndim, int n, const void *pos, const int32_t *rank,
                             const CellGrid &g, uint32_t *keys, int32_t *vals,
                             hipStream_t s);
-hipError_t radix_sort(int n, int bits, uint32_t *keys, int32_t *vals,
-                      uint32_t *tmp_keys, int32_t *tmp_vals, uint32_t *hist,
+hipError_t radix_sort(int n, int bits, uint32_t *&keys, int32_t *&vals,
+                      uint32_t *&tmp_keys, int32_t *&tmp_vals, uint32_t *hist,
                       hipStream_t s);
 template <typename T>
 hipError_t gather(int n, const int32_t *perm, const T *src, T *dst, hipStream_t s);
+// One permutation launch over every per-slot column: slot i of each
+// destination takes slot perm[i] of its source.
+constexpr int kPermD = 11, kPermI = 3;
+struct PermuteSet {
+    const void *pos_src[2];
+    void *pos_dst[2];
+    const double *d_src[kPermD];
+    double *d_dst[kPermD];
+    const int32_t *i_src[kPermI];
+    int32_t *i_dst[kPermI];
+    int nd, ni;
+};
+hipError_t launch_permute(int ndim, int n, const int32_t *perm, const PermuteSet &P,
+                          hipStream_t s);
 hipError_t launch_cell_bounds(int n, int ncell, const uint32_t *keys, int32_t *start,
                               hipStream_t s);
+// Outcome of one list build, stored by the build's last launch in pinned
+// host memory: the sum and the largest of the per-atom entry counts, the
+// flag word of the build (then cleared on the device), and for a
+// speculative tiled build the tile plan it ran against.
+struct BuildReport {
+    unsigned long long sum, max;
+    int ntiles, smax, bad;
+    uint32_t flag;
+};
+
 // Builds the full list (and xref = pos); stats[0] = sum of counts, stats[1] =
-// max count; part holds 2 * build_blocks(n) scratch words.
+// max count; part holds 2 * build_blocks(n) scratch words.  report: a
+// device-visible pinned BuildReport, or null.
 int build_blocks(int n);
 hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                              const int32_t *rank, const double *diam, const int32_t *type,
@@ -41,7 +66,7 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                              int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                              int32_t *ncount, uint16_t *lcount, void *xref,
                              unsigned long long *part, unsigned long long *stats,
-                             uint32_t *flag, hipStream_t s);
+                             uint32_t *flag, BuildReport *report, hipStream_t s);
 
 // Tiled layout (tiles.hpp).  launch_tiles fills tiles (cap_tiles records)
 // and info[0] = tiles, info[1] = largest staged stencil, info[2] != 0 when the
@@ -50,6 +75,10 @@ hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys
                         const int32_t *cstart, int cap_tiles, int4 *tiles, int2 *truns,
                         int *info, hipStream_t s);
 size_t tiled_build_lds(int smax);
+// spec == null: ntiles and smax are the plan's.  spec == the plan's info
+// (launch_tiles, not yet read by the host): ntiles and smax are bounds;
+// tiles past info[0] do nothing, and a tile with more than smax staged slots
+// (or none that fit) sets kFlagReplan instead of building.
 hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const int4 *tiles,
                               const void *pos, void *xw, const int32_t *rank,
                               const double *diam, const int32_t *type, int type_bits,
@@ -57,7 +86,8 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const int4 
                               const Box &box, double skin, double dmax, int tpa, int cap,
                               int npad, bool out32, void *nbr, int32_t *ncount,
                               uint16_t *lcount, void *xref, unsigned long long *part,
-                              unsigned long long *stats, uint32_t *flag, hipStream_t s);
+                              unsigned long long *stats, uint32_t *flag, const int *spec,
+                              BuildReport *report, hipStream_t s);
 hipError_t launch_tiled_to_global(int ndim, int ntiles, const int4 *tiles,
                                   const uint32_t *keys, const int32_t *cstart,
                                   const CellGrid &g, const uint16_t *nb16,

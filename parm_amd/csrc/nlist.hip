@@ -20,6 +20,8 @@
 //   k_tile_sizes<D>  staged slots of each tile, their maximum
 //   k_build_tiled<D> the tile's stencil staged in LDS, the same predicate
 //                    evaluated from LDS, 16-bit staged-slot entries
+#include <utility>
+
 #include "common.hpp"
 #include "kernels.hpp"
 #include "tiles.hpp"
@@ -170,6 +172,34 @@ __global__ __launch_bounds__(kBlock) void k_gather(int n, const int32_t *__restr
                                                    T *__restrict__ dst) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) dst[i] = src[perm[i]];
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_permute(int n, const int32_t *__restrict__ perm,
+                                                    PermuteSet P) {
+    using PT = typename PosStore<D>::T;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int p = perm[i];
+    // every load first, then every store: the gathers are in flight together
+    const PT x0 = static_cast<const PT *>(P.pos_src[0])[p];
+    const PT x1 = static_cast<const PT *>(P.pos_src[1])[p];
+    double d[kPermD];
+    int32_t q[kPermI];
+#pragma unroll
+    for (int c = 0; c < kPermD; c++)
+        if (c < P.nd) d[c] = P.d_src[c][p];
+#pragma unroll
+    for (int c = 0; c < kPermI; c++)
+        if (c < P.ni) q[c] = P.i_src[c][p];
+    static_cast<PT *>(P.pos_dst[0])[i] = x0;
+    static_cast<PT *>(P.pos_dst[1])[i] = x1;
+#pragma unroll
+    for (int c = 0; c < kPermD; c++)
+        if (c < P.nd) P.d_dst[c][i] = d[c];
+#pragma unroll
+    for (int c = 0; c < kPermI; c++)
+        if (c < P.ni) P.i_dst[c][i] = q[c];
 }
 
 // ---------------------------------------------------------- cell bounds
@@ -567,14 +597,36 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     double skin, double dmax, int tpa, int cap, int npad, void *__restrict__ nbr,
     int32_t *__restrict__ ncount, uint16_t *__restrict__ lcount,
     typename PosStore<D>::T *__restrict__ xref, unsigned long long *__restrict__ part,
-    uint32_t *__restrict__ flag) {
+    uint32_t *__restrict__ flag, const int *__restrict__ spec) {
     constexpr int NT = kWave * G;
     extern __shared__ float4 st_f[];
     uint32_t *st_e = reinterpret_cast<uint32_t *>(st_f + smax);  // OUT32 entries
     __shared__ TileLayout L;
     __shared__ unsigned long long red[G][2];
     __shared__ int mbits;
-    const int t = xcd_block(blockIdx.x, gridDim.x);
+    // tiles spread over the XCDs by the plan's tile count (a speculative
+    // grid is a bound: its workgroups past the count have no tile)
+    const int nt = spec ? spec[0] : (int)gridDim.x;
+    const int t = (int)blockIdx.x < nt ? xcd_block(blockIdx.x, nt) : nt;
+    if (spec) {
+        // speculative launch (grid and smax are bounds): the decision is the
+        // same for every thread of the workgroup
+        bool skip = t >= nt;
+        if (!skip) {
+            const int S0 = tiles[t].w;
+            if (S0 < 0 || S0 > smax) {
+                skip = true;
+                if (threadIdx.x == 0) atomicOr(flag, kFlagReplan);
+            }
+        }
+        if (skip) {
+            if (threadIdx.x == 0) {
+                part[2 * (size_t)blockIdx.x] = 0;
+                part[2 * (size_t)blockIdx.x + 1] = 0;
+            }
+            return;
+        }
+    }
     if (threadIdx.x == 0) mbits = 0;
     tile_layout<D>(tiles, t, keys, cstart, g, L);
     const int S = L.off[L.NR * L.W];
@@ -801,7 +853,8 @@ constexpr int kStatsThreads = 1024;
 
 __global__ __launch_bounds__(kStatsThreads) void k_build_stats(
     int nparts, const unsigned long long *__restrict__ part,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, uint32_t *__restrict__ flag,
+    const int *__restrict__ info, BuildReport *report) {
     __shared__ unsigned long long red[kStatsThreads / kWave][2];
     unsigned long long sum = 0, mx = 0;
 #pragma unroll 4
@@ -829,6 +882,19 @@ __global__ __launch_bounds__(kStatsThreads) void k_build_stats(
         }
         stats[0] = s2;
         stats[1] = m2;
+        const uint32_t fl = *flag;
+        *flag = 0;
+        if (report) {
+            __hip_atomic_store(&report->sum, s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&report->max, m2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&report->flag, fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&report->ntiles, info ? info[0] : 0, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&report->smax, info ? info[1] : 0, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&report->bad, info ? info[2] : 0, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -850,38 +916,26 @@ hipError_t launch_cell_keys(int ndim, int n, const void *pos, const int32_t *ran
 
 int radix_blocks(int n) { return (n + kRadixTile - 1) / kRadixTile; }
 
-// Sorts (keys, vals) by the low `bits` bits of the key; the result ends in
-// (keys, vals).  tmp_keys/tmp_vals are scratch of n entries; hist holds
-// 256 * radix_blocks(n) counters.
-hipError_t radix_sort(int n, int bits, uint32_t *keys, int32_t *vals,
-                      uint32_t *tmp_keys, int32_t *tmp_vals, uint32_t *hist,
+// Sorts (keys, vals) by the low `bits` bits of the key.  The passes
+// ping-pong between (keys, vals) and (tmp_keys, tmp_vals); after an odd
+// number of passes the two pairs of pointers are swapped, so the result is
+// in (keys, vals) either way.  hist holds 256 * radix_blocks(n) counters.
+hipError_t radix_sort(int n, int bits, uint32_t *&keys, int32_t *&vals,
+                      uint32_t *&tmp_keys, int32_t *&tmp_vals, uint32_t *hist,
                       hipStream_t s) {
     if (n <= 1) return hipSuccess;
     const int nb = radix_blocks(n);
-    uint32_t *ka = keys, *kb = tmp_keys;
-    int32_t *va = vals, *vb = tmp_vals;
     int passes = (bits + 7) / 8;
     if (passes < 1) passes = 1;
     for (int p = 0; p < passes; p++) {
         int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, s, n, shift,
-                           ka, hist);
+                           keys, hist);
         hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, 256 * nb, hist);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, s, n, shift,
-                           ka, va, hist, kb, vb);
-        uint32_t *tk = ka;
-        ka = kb;
-        kb = tk;
-        int32_t *tv = va;
-        va = vb;
-        vb = tv;
-    }
-    if (ka != keys) {
-        hipError_t e = hipMemcpyAsync(keys, ka, sizeof(uint32_t) * n,
-                                      hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return e;
-        e = hipMemcpyAsync(vals, va, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return e;
+                           keys, vals, hist, tmp_keys, tmp_vals);
+        std::swap(keys, tmp_keys);
+        std::swap(vals, tmp_vals);
     }
     return hipGetLastError();
 }
@@ -902,6 +956,16 @@ template hipError_t gather<double2>(int, const int32_t *, const double2 *, doubl
 template hipError_t gather<double4>(int, const int32_t *, const double4 *, double4 *,
                                     hipStream_t);
 
+hipError_t launch_permute(int ndim, int n, const int32_t *perm, const PermuteSet &P,
+                          hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_permute<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n, perm, P);
+    else
+        hipLaunchKernelGGL(k_permute<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n, perm, P);
+    return hipGetLastError();
+}
+
 hipError_t launch_cell_bounds(int n, int ncell, const uint32_t *keys, int32_t *start,
                               hipStream_t s) {
     hipLaunchKernelGGL(k_cell_bounds, dim3(nblk(n + 1)), dim3(kBlock), 0, s, n,
@@ -921,7 +985,7 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                              int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                              int32_t *ncount, uint16_t *lcount, void *xref,
                              unsigned long long *part, unsigned long long *stats,
-                             uint32_t *flag, hipStream_t s) {
+                             uint32_t *flag, BuildReport *report, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int nb = build_blocks(n);
     if (ndim == 3) {
@@ -939,7 +1003,8 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                            keys, cstart, g, box, skin, dmax, type_bits, tpa, cap, npad,
                            nbr, ncount, lcount, (double2 *)xref, part, flag);
     }
-    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, nb, part, stats);
+    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, nb, part, stats,
+                       flag, (const int *)nullptr, report);
     return hipGetLastError();
 }
 
@@ -972,14 +1037,15 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
                               const Box &box, double skin, double dmax, int tpa, int cap,
                               int npad, bool out32, void *nbr, int32_t *ncount,
                               uint16_t *lcount, void *xref, unsigned long long *part,
-                              unsigned long long *stats, uint32_t *flag, hipStream_t s) {
+                              unsigned long long *stats, uint32_t *flag, const int *spec,
+                              BuildReport *report, hipStream_t s) {
     if (n <= 0 || ntiles <= 0) return hipSuccess;
     const size_t lds = (size_t)smax * (sizeof(float4) + (out32 ? sizeof(uint32_t) : 0));
 #define PARM_BT(DD, PT, GG, O32)                                                              \
     hipLaunchKernelGGL((k_build_tiled<DD, GG, kTiledUnroll, O32>), dim3(ntiles),               \
                        dim3(kWave * GG), lds, s, tiles, smax, (const PT *)pos, (const PT *)xw, \
                        rank, diam, type, type_bits, keys, cstart, g, box, skin, dmax, tpa,     \
-                       cap, npad, nbr, ncount, lcount, (PT *)xref, part, flag)
+                       cap, npad, nbr, ncount, lcount, (PT *)xref, part, flag, spec)
 #define PARM_BT_G(DD, PT, O32) PARM_BT(DD, PT, kTiledLanes, O32);
     if (ndim == 3) {
         hipLaunchKernelGGL(k_wrap<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
@@ -1000,7 +1066,8 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
     }
 #undef PARM_BT_G
 #undef PARM_BT
-    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, ntiles, part, stats);
+    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, ntiles, part, stats,
+                       flag, spec, report);
     return hipGetLastError();
 }
 
