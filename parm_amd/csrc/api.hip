@@ -110,7 +110,7 @@ struct parm_system {
 
     // neighbour list
     bool list_cfg = false;
-    double skin = 0.0, dmax = 0.0;
+    double skin = 0.0, dmax = 0.0, dmin = 0.0;  // list diameters of the members
     int64_t nmem = 0;
     bool ignorechanged = true;
     uint32_t which = 0;
@@ -378,12 +378,13 @@ int build_list(parm_system *s, const void *at) {
     hipStream_t st = s->stream;
     if (s->tile_ok) {
         // the tiled build writes either list layout
-        PARM_HIP(launch_build_tiled(s->ndim, s->n, s->ntiles, s->smax, s->tiles, at, s->xwrap,
+        PARM_HIP(launch_build_tiled(s->ndim, s->n, s->ntiles, s->smax, s->tiles, s->truns, at,
+                                    s->xwrap,
                                     s->rank, s->diam, s->type, s->type_bits, s->keys,
                                     s->cstart, s->grid, s->box, s->skin, s->dmax, s->tpa,
                                     s->cap, s->npad, !s->tiled, s->nbr, s->ncount, s->lcount,
                                     s->xref, s->bpart, s->stats, s->dflag + kSlotBuild,
-                                    nullptr, s->hrep_dev, st));
+                                    nullptr, s->dmin == s->dmax, s->hrep_dev, st));
     } else {
         PARM_HIP(launch_build_list(s->ndim, s->n, at, s->xwrap, s->rank, s->diam, s->type,
                                    s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
@@ -440,11 +441,13 @@ int build_speculative(parm_system *s) {
     const int limit = (int)(s->lds_max / tiled_build_lds(1));
     const int bound = std::min(limit, s->spec_smax + std::max(s->spec_smax / 8, 64));
     const bool tiled = s->layout_pref != 2;
-    PARM_HIP(launch_build_tiled(s->ndim, s->n, grid, bound, s->tiles, s->pos, s->xwrap, s->rank,
+    PARM_HIP(launch_build_tiled(s->ndim, s->n, grid, bound, s->tiles, s->truns, s->pos, s->xwrap,
+                                s->rank,
                                 s->diam, s->type, s->type_bits, s->keys, s->cstart, s->grid,
                                 s->box, s->skin, s->dmax, s->tpa, s->cap, s->npad, !tiled,
                                 s->nbr, s->ncount, s->lcount, s->xref, s->bpart, s->stats,
-                                s->dflag + kSlotBuild, s->tinfo, s->hrep_dev, s->stream));
+                                s->dflag + kSlotBuild, s->tinfo, s->dmin == s->dmax,
+                                s->hrep_dev, s->stream));
     BuildReport r{};
     PARM_CHECK(read_report(s, &r));
     accept_tiles(s, r.ntiles, r.smax, r.bad);
@@ -1074,7 +1077,7 @@ int parm_nlist_set(parm_system *s, double skin, int64_t nmembers, const int32_t 
         return fail(PARM_E_INVALID, "null member/diameter array");
     std::vector<int32_t> rank(s->n, -1);
     std::vector<double> dia(s->n, 0.0);
-    double dmax = 0.0;
+    double dmax = 0.0, dmin = 0.0;
     for (int64_t k = 0; k < nmembers; k++) {
         int32_t a = member[k];
         if (a < 0 || a >= s->n)
@@ -1087,6 +1090,7 @@ int parm_nlist_set(parm_system *s, double skin, int64_t nmembers, const int32_t 
         rank[a] = (int32_t)k;
         dia[a] = diameter[k];
         dmax = std::max(dmax, diameter[k]);
+        dmin = k == 0 ? diameter[k] : std::min(dmin, diameter[k]);
     }
     PARM_CHECK(set_device(s));
     if (s->n > 0) {
@@ -1095,6 +1099,7 @@ int parm_nlist_set(parm_system *s, double skin, int64_t nmembers, const int32_t 
     }
     s->skin = skin;
     s->dmax = dmax;
+    s->dmin = dmin;
     s->nmem = nmembers;
     s->list_cfg = true;
     s->ignorechanged = true;

@@ -78,16 +78,17 @@ size_t tiled_build_lds(int smax);
 // spec == null: ntiles and smax are the plan's.  spec == the plan's info
 // (launch_tiles, not yet read by the host): ntiles and smax are bounds;
 // tiles past info[0] do nothing, and a tile with more than smax staged slots
-// (or none that fit) sets kFlagReplan instead of building.
+// (or none that fit) sets kFlagReplan instead of building.  uni: every list
+// member has the same diameter.
 hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const int4 *tiles,
-                              const void *pos, void *xw, const int32_t *rank,
+                              const int2 *truns, const void *pos, void *xw, const int32_t *rank,
                               const double *diam, const int32_t *type, int type_bits,
                               const uint32_t *keys, const int32_t *cstart, const CellGrid &g,
                               const Box &box, double skin, double dmax, int tpa, int cap,
                               int npad, bool out32, void *nbr, int32_t *ncount,
                               uint16_t *lcount, void *xref, unsigned long long *part,
                               unsigned long long *stats, uint32_t *flag, const int *spec,
-                              BuildReport *report, hipStream_t s);
+                              int uni, BuildReport *report, hipStream_t s);
 hipError_t launch_tiled_to_global(int ndim, int ntiles, const int4 *tiles,
                                   const uint32_t *keys, const int32_t *cstart,
                                   const CellGrid &g, const uint16_t *nb16,

@@ -20,6 +20,7 @@
 //   k_tile_sizes<D>  staged slots of each tile, their maximum
 //   k_build_tiled<D> the tile's stencil staged in LDS, the same predicate
 //                    evaluated from LDS, 16-bit staged-slot entries
+#include <type_traits>
 #include <utility>
 
 #include "common.hpp"
@@ -523,13 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_sizes(int cap_tiles, CellGrid g
                 const int cz = D == 3 ? T.z / g.nc[1] : 0;
                 const int NR = stencil_ny<D>(g) * stencil_nz<D>(g);
                 // per stencil row: the cells before / after the wrap along x
-                int ustar = W;
-                if (g.nc[0] >= 3 && !xfull) {
-                    if (xa - 1 < 0)
-                        ustar = 1;
-                    else if (xa - 1 + W > g.nc[0])
-                        ustar = g.nc[0] - (xa - 1);
-                }
+                const int ustar = stencil_ustar(g, xa, W, xfull);
                 for (int k = 0; k < NR; k++) {
                     for (int h = 0; h < 2; h++) {
                         const int u0 = h == 0 ? 0 : ustar, u1 = h == 0 ? ustar : W;
@@ -565,6 +560,31 @@ __global__ __launch_bounds__(kBlock) void k_tile_sizes(int cap_tiles, CellGrid g
     }
 }
 
+// Inclusive prefix sum of v over the G consecutive lanes of a group (lane
+// `sub` of the group; glast = the wave lane of the group's last lane) and
+// the group total.  For G = 4 the groups are the DPP quads: quad_perm moves
+// (VALU, no LDS crossbar) shift the values up by 1 and 2 lanes and
+// broadcast lane 3.
+template <int G>
+__device__ __forceinline__ void group_scan(int v, int sub, int glast, int &incl, int &tot) {
+    if constexpr (G == 4) {
+        incl = v;
+        const int y1 = __builtin_amdgcn_update_dpp(0, incl, 0x90, 0xF, 0xF, false);  // [0,0,1,2]
+        if (sub >= 1) incl += y1;
+        const int y2 = __builtin_amdgcn_update_dpp(0, incl, 0x44, 0xF, 0xF, false);  // [0,1,0,1]
+        if (sub >= 2) incl += y2;
+        tot = __builtin_amdgcn_update_dpp(0, incl, 0xFF, 0xF, 0xF, false);  // [3,3,3,3]
+    } else {
+        incl = v;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (sub >= o) incl += y;
+        }
+        tot = __shfl(incl, glast);
+    }
+}
+
 // Tiled list build: one workgroup per tile, G lanes per atom.  The tile's
 // stencil is staged in LDS as FP32 offsets from the tile's first atom (its
 // wrapped position), taken from the wrapped positions shifted by the stencil
@@ -587,10 +607,17 @@ __global__ __launch_bounds__(kBlock) void k_tile_sizes(int cap_tiles, CellGrid g
 // count / tpa entries.  lcount[i * tpa + l] is lane l's count.
 // OUT32: write untiled entries ((j << type_bits) | type_j, k_build_list's
 // layout) instead of staged slots, for systems whose force kernel runs on the
-// untiled list.
-template <int D, int G, int U, bool OUT32>
+// untiled list.  UNI: every member has the same list diameter; the staged
+// slot then holds the offsets only (one cutoff for every candidate).
+template <int D, bool UNI>
+struct BuildSlot {
+    static constexpr int W = UNI ? D : D + 1;  // floats per slot
+    float v[W];
+};
+
+template <int D, int G, int U, bool OUT32, bool UNI>
 __global__ __launch_bounds__(kWave *G) void k_build_tiled(
-    const TileRec *__restrict__ tiles, int smax, const typename PosStore<D>::T *__restrict__ pos,
+    const TileRec *__restrict__ tiles, const int2 *__restrict__ truns, int smax, const typename PosStore<D>::T *__restrict__ pos,
     const typename PosStore<D>::T *__restrict__ xw, const int32_t *__restrict__ rank,
     const double *__restrict__ diam, const int32_t *__restrict__ type, int type_bits,
     const uint32_t *__restrict__ keys, const int32_t *__restrict__ cstart, CellGrid g, Box box,
@@ -599,9 +626,13 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     typename PosStore<D>::T *__restrict__ xref, unsigned long long *__restrict__ part,
     uint32_t *__restrict__ flag, const int *__restrict__ spec) {
     constexpr int NT = kWave * G;
-    extern __shared__ float4 st_f[];
+    using Slot = BuildSlot<D, UNI>;
+    extern __shared__ float st_raw[];
+    Slot *st_f = reinterpret_cast<Slot *>(st_raw);
     uint32_t *st_e = reinterpret_cast<uint32_t *>(st_f + smax);  // OUT32 entries
     __shared__ TileLayout L;
+    __shared__ int2 runs[kTileRuns];
+    __shared__ double rimg[kTileRuns][3];
     __shared__ unsigned long long red[G][2];
     __shared__ int mbits;
     // tiles spread over the XCDs by the plan's tile count (a speculative
@@ -630,25 +661,34 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     if (threadIdx.x == 0) mbits = 0;
     tile_layout<D>(tiles, t, keys, cstart, g, L);
     const int S = L.off[L.NR * L.W];
+    // the stencil's runs (contiguous atoms, one periodic image each)
+    if (threadIdx.x < kTileRuns) {
+        const int r = threadIdx.x;
+        runs[r] = truns[(size_t)t * kTileRuns + r];
+        int img[3];
+        run_image<D>(L, g, r, img);
+#pragma unroll
+        for (int k = 0; k < 3; k++) rimg[r][k] = k < D ? img[k] * box.L[k] : 0.0;
+    }
     Vec<D> wo;
 #pragma unroll
     for (int k = 0; k < D; k++) wo.c[k] = 0.0;
     if (S > 0) wo = PosStore<D>::load(xw[L.a0]);
+    __syncthreads();
     float mloc = 0.0f;
+    int r = 0;  // run of slot s (s grows along the loop: a monotone walk)
     for (int s = threadIdx.x; s < S; s += NT) {
-        const int c = slot_cell(L, s);
-        int img[3];
-        (void)stencil_cell<D>(g, L.cy, L.cz, L.xa, c / L.W, c % L.W, img);
-        const int j = L.g0[c] + (s - L.off[c]);
+        while (r + 1 < kTileRuns && runs[r + 1].y <= s) r++;
+        const int j = runs[r].x + (s - runs[r].y);
         const Vec<D> w = PosStore<D>::load(xw[j]);
-        float r[3] = {0.0f, 0.0f, 0.0f};
+        Slot sl;
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            r[k] = (float)((w.c[k] + img[k] * box.L[k]) - wo.c[k]);
-            mloc = fmaxf(mloc, fabsf(r[k]));
+            sl.v[k] = (float)((w.c[k] + rimg[r][k]) - wo.c[k]);
+            mloc = fmaxf(mloc, fabsf(sl.v[k]));
         }
-        const float dj = (float)diam[j];
-        st_f[s] = D == 3 ? float4{r[0], r[1], r[2], dj} : float4{r[0], r[1], dj, 0.0f};
+        if constexpr (!UNI) sl.v[D] = (float)diam[j];
+        st_f[s] = sl;
         if (OUT32) st_e[s] = ((uint32_t)j << type_bits) | (uint32_t)type[j];
     }
     if (mloc > 0.0f) atomicMax(&mbits, __float_as_int(mloc));  // >= 0: int order
@@ -702,74 +742,95 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                 self = L.off[cc] + (i - L.g0[cc]);
             }
             int cnt = 0;  // the atom's entries so far (same in all its lanes)
-            for (int kr = 0; kr < 2 * L.NR; kr++) {
-                int lo, hi, lo1, hi1;
-                stencil_ranges(L, g, kr >> 1, cx, lo, hi, lo1, hi1);
-                if (kr & 1) {
-                    lo = lo1;
-                    hi = hi1;
-                }
-                for (int base = lo; base < hi; base += G * U) {
-                    float4 pw[U];
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const int s = base + u * G + sub;
-                        pw[u] = st_f[s < hi ? s : lo];
+            // The candidate scan, compiled with and without FOLD (a minimum
+            // image taken explicitly along some dimension).  With UNI the
+            // one cutoff is the same FP32 value the per-pair formula gives.
+            const bool anyfold = foldk[0] || (D >= 2 && foldk[1]) || (D == 3 && foldk[2]);
+            const float cu = fmaf(0.5f, dif + dif, skf);
+            const float c2u = cu * cu;
+            auto scan = [&](auto FOLDC) {
+                constexpr bool FOLD = decltype(FOLDC)::value;
+                for (int kr = 0; kr < 2 * L.NR; kr++) {
+                    int lo, hi, lo1, hi1;
+                    stencil_ranges(L, g, kr >> 1, cx, lo, hi, lo1, hi1);
+                    if (kr & 1) {
+                        lo = lo1;
+                        hi = hi1;
                     }
-                    bool pass[U];
-                    int np = 0;
+                    for (int base = lo; base < hi; base += G * U) {
+                        Slot pw[U];
 #pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const int s = base + u * G + sub;
-                        pass[u] = false;
-                        if (s < hi && s != self) {
-                            const float pc[3] = {pw[u].x, pw[u].y, pw[u].z};
-                            const float djf = D == 3 ? pw[u].w : pw[u].z;
+                        for (int u = 0; u < U; u++) {
+                            const int s = base + u * G + sub;
+                            pw[u] = st_f[s < hi ? s : lo];
+                        }
+                        bool pass[U], unsure[U];
+                        bool anyu = false;
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int s = base + u * G + sub;
                             float d2 = 0.0f;
 #pragma unroll
                             for (int k2 = 0; k2 < D; k2++) {
-                                float dk = ri[k2] - pc[k2];
-                                if (foldk[k2]) dk -= Lf[k2] * rintf(dk * iLf[k2]);
+                                float dk = ri[k2] - pw[u].v[k2];
+                                if (FOLD && foldk[k2]) dk -= Lf[k2] * rintf(dk * iLf[k2]);
                                 d2 = fmaf(dk, dk, d2);
                             }
-                            const float cc = fmaf(0.5f, dif + djf, skf);
-                            const float c2 = cc * cc;
-                            if (d2 < c2 - tol) {
-                                pass[u] = true;
-                            } else if (d2 <= c2 + tol) {
-                                const int cj = slot_cell(L, s);
-                                const int j = L.g0[cj] + (s - L.off[cj]);
-                                pass[u] = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
-                                                     diam[j], box, skin);
+                            float c2 = c2u;
+                            if constexpr (!UNI) {
+                                const float cc = fmaf(0.5f, dif + pw[u].v[UNI ? 0 : D], skf);
+                                c2 = cc * cc;
+                            }
+                            const bool ok = s < hi && s != self;
+                            pass[u] = ok && d2 < c2 - tol;
+                            unsure[u] = ok && !(d2 < c2 - tol) && d2 <= c2 + tol;
+                            anyu |= unsure[u];
+                        }
+                        // candidates within the tolerance of the cutoff: the
+                        // reference's exact predicate (one wave-uniform branch)
+                        if (__ballot(anyu)) {
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                if (unsure[u]) {
+                                    const int s = base + u * G + sub;
+                                    const int cj = slot_cell(L, s);
+                                    const int j = L.g0[cj] + (s - L.off[cj]);
+                                    pass[u] = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
+                                                         diam[j], box, skin);
+                                }
                             }
                         }
-                        np += pass[u] ? 1 : 0;
-                    }
-                    // exclusive prefix of the round's passes over the group
-                    int incl = np;
+                        int np = 0;
 #pragma unroll
-                    for (int o = 1; o < G; o <<= 1) {
-                        const int y = __shfl_up(incl, o);
-                        if (sub >= o) incl += y;
-                    }
-                    int q = cnt + incl - np;
-                    cnt += __shfl(incl, glast);
+                        for (int u = 0; u < U; u++) np += pass[u] ? 1 : 0;
+                        // exclusive prefix of the round's passes over the group
+                        // (the G lanes of one atom), and the group total
+                        int incl, tot;
+                        group_scan<G>(np, sub, glast, incl, tot);
+                        int q = cnt + incl - np;
+                        cnt += tot;
 #pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        if (!pass[u]) continue;
-                        if (q < cap) {
-                            const int s = base + u * G + sub;
-                            if (OUT32)
-                                static_cast<uint32_t *>(
-                                    nbr)[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] = st_e[s];
-                            else
-                                static_cast<uint16_t *>(nbr)[tiled_entry(q, i, tpa, npad)] =
-                                    (uint16_t)s;
+                        for (int u = 0; u < U; u++) {
+                            if (!pass[u]) continue;
+                            if (q < cap) {
+                                const int s = base + u * G + sub;
+                                if (OUT32)
+                                    static_cast<uint32_t *>(
+                                        nbr)[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] =
+                                        st_e[s];
+                                else
+                                    static_cast<uint16_t *>(nbr)[tiled_entry(q, i, tpa, npad)] =
+                                        (uint16_t)s;
+                            }
+                            q++;
                         }
-                        q++;
                     }
                 }
-            }
+            };
+            if (anyfold)
+                scan(std::true_type{});
+            else
+                scan(std::false_type{});
             if (sub == 0) {
                 const int cc = cnt < cap ? cnt : cap;
                 ncount[i] = cc;
@@ -1026,45 +1087,52 @@ hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys
 }
 
 constexpr int kTiledLanes = 4;   // lanes per atom in k_build_tiled
-constexpr int kTiledUnroll = 4;
+constexpr int kTiledUnroll = 8;
 
 size_t tiled_build_lds(int smax) { return (size_t)smax * (sizeof(float4) + sizeof(uint32_t)); }
 
 hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileRec *tiles,
-                              const void *pos, void *xw, const int32_t *rank,
+                              const int2 *truns, const void *pos, void *xw, const int32_t *rank,
                               const double *diam, const int32_t *type, int type_bits,
                               const uint32_t *keys, const int32_t *cstart, const CellGrid &g,
                               const Box &box, double skin, double dmax, int tpa, int cap,
                               int npad, bool out32, void *nbr, int32_t *ncount,
                               uint16_t *lcount, void *xref, unsigned long long *part,
                               unsigned long long *stats, uint32_t *flag, const int *spec,
-                              BuildReport *report, hipStream_t s) {
+                              int uni, BuildReport *report, hipStream_t s) {
     if (n <= 0 || ntiles <= 0) return hipSuccess;
-    const size_t lds = (size_t)smax * (sizeof(float4) + (out32 ? sizeof(uint32_t) : 0));
-#define PARM_BT(DD, PT, GG, O32)                                                              \
-    hipLaunchKernelGGL((k_build_tiled<DD, GG, kTiledUnroll, O32>), dim3(ntiles),               \
-                       dim3(kWave * GG), lds, s, tiles, smax, (const PT *)pos, (const PT *)xw, \
-                       rank, diam, type, type_bits, keys, cstart, g, box, skin, dmax, tpa,     \
-                       cap, npad, nbr, ncount, lcount, (PT *)xref, part, flag, spec)
-#define PARM_BT_G(DD, PT, O32) PARM_BT(DD, PT, kTiledLanes, O32);
+    const size_t slot = (size_t)(uni ? ndim : ndim + 1) * sizeof(float);
+    const size_t lds = (size_t)smax * (slot + (out32 ? sizeof(uint32_t) : 0));
+#define PARM_BT(DD, PT, O32, UU)                                                             \
+    hipLaunchKernelGGL((k_build_tiled<DD, kTiledLanes, kTiledUnroll, O32, UU>), dim3(ntiles), \
+                       dim3(kWave * kTiledLanes), lds, s, tiles, truns, smax, (const PT *)pos, \
+                       (const PT *)xw, rank, diam, type, type_bits, keys, cstart, g, box,     \
+                       skin, dmax, tpa, cap, npad, nbr, ncount, lcount, (PT *)xref, part,     \
+                       flag, spec)
+#define PARM_BT_U(DD, PT, O32)       \
+    if (uni) {                      \
+        PARM_BT(DD, PT, O32, true);  \
+    } else {                        \
+        PARM_BT(DD, PT, O32, false); \
+    }
     if (ndim == 3) {
         hipLaunchKernelGGL(k_wrap<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double4 *)pos, g, box, (double4 *)xw);
         if (out32) {
-            PARM_BT_G(3, double4, true)
+            PARM_BT_U(3, double4, true)
         } else {
-            PARM_BT_G(3, double4, false)
+            PARM_BT_U(3, double4, false)
         }
     } else {
         hipLaunchKernelGGL(k_wrap<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double2 *)pos, g, box, (double2 *)xw);
         if (out32) {
-            PARM_BT_G(2, double2, true)
+            PARM_BT_U(2, double2, true)
         } else {
-            PARM_BT_G(2, double2, false)
+            PARM_BT_U(2, double2, false)
         }
     }
-#undef PARM_BT_G
+#undef PARM_BT_U
 #undef PARM_BT
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, ntiles, part, stats,
                        flag, spec, report);

@@ -114,6 +114,29 @@ __device__ __forceinline__ int stencil_x(const CellGrid &g, int &xa, int xb, int
     return 1;
 }
 
+// First stencil column (u) past the periodic wrap along x, for a tile with
+// first x cell xa and width W (W: no wrap).  Within each stencil row the
+// columns before it and from it on are the row's two runs; each run is a
+// contiguous range of cells with one periodic image.
+__device__ __forceinline__ int stencil_ustar(const CellGrid &g, int xa, int W, int xfull) {
+    if (g.nc[0] < 3 || xfull) return W;
+    if (xa - 1 < 0) return 1;
+    if (xa - 1 + W > g.nc[0]) return g.nc[0] - (xa - 1);
+    return W;
+}
+
+// Periodic image (box lengths per dimension) of run r of a tile: stencil row
+// r / 2, columns before (r even) or from (r odd) the wrap.  Zero for an
+// empty run.
+template <int D>
+__device__ __forceinline__ void run_image(const TileLayout &L, const CellGrid &g, int r,
+                                          int (&img)[3]) {
+    img[0] = img[1] = img[2] = 0;
+    const int ustar = stencil_ustar(g, L.xa, L.W, L.xfull);
+    const int u0 = (r & 1) ? ustar : 0;
+    if (r / 2 < L.NR && u0 < L.W) (void)stencil_cell<D>(g, L.cy, L.cz, L.xa, r / 2, u0, img);
+}
+
 // Staged ranges of stencil row k seen by an atom in x cell cx: [lo0, hi0)
 // and [lo1, hi1) (the second one empty unless a full row wraps).
 __device__ __forceinline__ void stencil_ranges(const TileLayout &L, const CellGrid &g,
