@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
                             (d2 <= c2 * (1.0 + 1e-6) && in_list<D>(xi, xj, di, dj, G.box, skin));
                         if (pass) {
                             if (cnt < cap)
-                                nbr[((size_t)(cnt / tpa) * npad + i) * tpa + cnt % tpa] =
+                                nbr[untiled_entry(cnt, i, tpa, npad)] =
                                     ((uint32_t)j << type_bits) | (uint32_t)A.type[j];
                             cnt++;
                         }

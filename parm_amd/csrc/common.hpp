@@ -56,6 +56,17 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// log2 of the lanes per atom (a power of two: 1, 2, 4 or 8), so that list
+// entry positions take shifts and masks instead of integer divisions.
+__device__ __forceinline__ int tpa_shift(int tpa) { return 31 - __clz(tpa); }
+
+// Position of entry q of slot i in the 32-bit (untiled) list:
+// ((q / tpa) * npad + i) * tpa + q % tpa.
+__device__ __forceinline__ size_t untiled_entry(int q, int i, int tpa, int npad) {
+    const int ts = tpa_shift(tpa);
+    return (((size_t)(q >> ts) * npad + i) << ts) + (q & (tpa - 1));
+}
+
 // Box: edge lengths, their halves and inverses.
 struct Box {
     double L[3];

@@ -384,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
                             if (pass) {
                                 const int q = cnt + __popcll(m & below);
                                 if (q < cap)
-                                    nbr[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] =
+                                    nbr[untiled_entry(q, i, tpa, npad)] =
                                         ((uint32_t)j << type_bits) | (uint32_t)type[j];
                             }
                             cnt += __popcll(m);
@@ -816,7 +816,7 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                                 const int s = base + u * G + sub;
                                 if (OUT32)
                                     static_cast<uint32_t *>(
-                                        nbr)[((size_t)(q / tpa) * npad + i) * tpa + q % tpa] =
+                                        nbr)[untiled_entry(q, i, tpa, npad)] =
                                         st_e[s];
                                 else
                                     static_cast<uint16_t *>(nbr)[tiled_entry(q, i, tpa, npad)] =

@@ -267,8 +267,9 @@ __device__ __forceinline__ int slot_run(const int2 *runs, int s) {
 // 16-bit tiled list: word ((m / 2) * npad + i) * tpa + q % tpa holds lane
 // entries m = 2w (low half) and 2w + 1 (high half).
 __device__ __forceinline__ size_t tiled_entry(int q, int i, int tpa, int npad) {
-    const int m = q / tpa;
-    return ((((size_t)(m >> 1) * npad + i) * tpa + (q - m * tpa)) << 1) + (m & 1);
+    const int ts = tpa_shift(tpa);
+    const int m = q >> ts;
+    return (((((size_t)(m >> 1) * npad + i) << ts) + (q & (tpa - 1))) << 1) + (m & 1);
 }
 
 }  // namespace parm
