@@ -965,7 +965,7 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
         (rc = dalloc(&s->stage_small, n)) || (rc = dalloc(&s->keys, n)) ||
         (rc = dalloc(&s->tkeys, n)) || (rc = dalloc(&s->vals, n)) ||
         (rc = dalloc(&s->tvals, n)) ||
-        (rc = dalloc(&s->hist, (size_t)256 * radix_blocks((int)n))) ||
+        (rc = dalloc(&s->hist, (size_t)256 * (radix_blocks((int)n) + 1))) ||
         (rc = dalloc(&s->stats, 2)) ||
         (rc = dalloc(&s->bpart, 2 * (size_t)build_blocks((int)n))) ||
         (rc = dalloc(&s->part, (size_t)kRedWidth * force_blocks((int)n, 8))) ||

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(
 // digit d in tile b; its exclusive scan is the output offset of (d, b), which
 // keeps equal digits in input order across tiles.  Within a tile the scatter
 // ranks equal digits in input order with 64-lane ballots.
-constexpr int kRadixItems = 16;
+constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kBlock * kRadixItems;
 
 __global__ __launch_bounds__(kBlock) void k_radix_hist(
@@ -79,56 +79,67 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(
     hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of m entries (m a multiple of 4) by one workgroup of 1024
-// threads: each thread owns a contiguous chunk (a multiple of 4 entries) that
-// it reads and writes with 16-byte accesses, so its loads are in flight
-// together; the chunk totals are scanned in LDS.
-__global__ __launch_bounds__(1024) void k_radix_scan(int m,
-                                                     uint32_t *__restrict__ a) {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    const int per = ((m + 1023) / 1024 + 3) / 4 * 4;
-    const int lo = min(m, t * per), hi = min(m, lo + per);
-    uint4 *a4 = reinterpret_cast<uint4 *>(a);
-    uint32_t s = 0;
-    for (int i = lo; i < hi; i += 4) {
-        const uint4 v = a4[i / 4];
-        s += v.x + v.y + v.z + v.w;
+// Exclusive scan of one value per thread over a workgroup of NT threads;
+// total = the sum over the workgroup.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t &total) {
+    __shared__ uint32_t wsum[NT / kWave];
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
     }
-    part[t] = s;
+    if (lane == kWave - 1) wsum[w] = incl;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = (t >= off) ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (int q = 0; q < NT / kWave; q++) {
+        const uint32_t ws = wsum[q];
+        if (q < w) before += ws;
+        total += ws;
     }
-    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
-    for (int i = lo; i < hi; i += 4) {
-        uint4 v = a4[i / 4];
-        uint4 o;
-        o.x = run;
-        run += v.x;
-        o.y = run;
-        run += v.y;
-        o.z = run;
-        run += v.z;
-        o.w = run;
-        run += v.w;
-        a4[i / 4] = o;
+    __syncthreads();
+    return before + incl - v;
+}
+
+// One workgroup per digit d: exclusive scan of the digit's per-tile counts
+// hist[d * nb + b] in place (each thread a contiguous chunk), and
+// dtot[d] = the digit's total.  The scatter adds the digits' exclusive scan.
+__global__ __launch_bounds__(kBlock) void k_radix_scan(int nb, uint32_t *__restrict__ hist,
+                                                     uint32_t *__restrict__ dtot) {
+    uint32_t *h = hist + (size_t)blockIdx.x * nb;
+    const int per = (nb + kBlock - 1) / kBlock;
+    const int lo = min(nb, (int)threadIdx.x * per), hi = min(nb, lo + per);
+    uint32_t s = 0;
+    for (int i = lo; i < hi; i++) s += h[i];
+    uint32_t total = 0;
+    uint32_t run = block_excl_scan_u32<kBlock>(s, total);
+    for (int i = lo; i < hi; i++) {
+        const uint32_t v = h[i];
+        h[i] = run;
+        run += v;
     }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = total;
 }
 
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     int n, int shift, const uint32_t *__restrict__ keys_in,
     const int32_t *__restrict__ vals_in, const uint32_t *__restrict__ offs,
-    uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out) {
+    const uint32_t *__restrict__ dtot, uint32_t *__restrict__ keys_out,
+    int32_t *__restrict__ vals_out) {
+    static_assert(kBlock == 256, "one thread per digit");
     __shared__ uint32_t base[256];
     __shared__ uint32_t wcnt[kBlock / kWave][256];
     const int t = threadIdx.x;
     const int w = t / kWave;
     const int lane = t % kWave;
-    base[t] = offs[t * gridDim.x + blockIdx.x];
+    // first output slot of digit t in this tile: the totals of the smaller
+    // digits plus the counts of digit t in the earlier tiles
+    uint32_t all = 0;
+    base[t] = block_excl_scan_u32<kBlock>(dtot[t], all) + offs[t * gridDim.x + blockIdx.x];
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int tile = blockIdx.x * kRadixTile;
     for (int c = 0; c < kRadixItems; c++) {
@@ -980,7 +991,8 @@ int radix_blocks(int n) { return (n + kRadixTile - 1) / kRadixTile; }
 // Sorts (keys, vals) by the low `bits` bits of the key.  The passes
 // ping-pong between (keys, vals) and (tmp_keys, tmp_vals); after an odd
 // number of passes the two pairs of pointers are swapped, so the result is
-// in (keys, vals) either way.  hist holds 256 * radix_blocks(n) counters.
+// in (keys, vals) either way.  hist holds 256 * (radix_blocks(n) + 1)
+// counters (the last 256: digit totals).
 hipError_t radix_sort(int n, int bits, uint32_t *&keys, int32_t *&vals,
                       uint32_t *&tmp_keys, int32_t *&tmp_vals, uint32_t *hist,
                       hipStream_t s) {
@@ -992,9 +1004,10 @@ hipError_t radix_sort(int n, int bits, uint32_t *&keys, int32_t *&vals,
         int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, s, n, shift,
                            keys, hist);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, 256 * nb, hist);
+        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(kBlock), 0, s, nb, hist,
+                           hist + (size_t)256 * nb);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, s, n, shift,
-                           keys, vals, hist, tmp_keys, tmp_vals);
+                           keys, vals, hist, hist + (size_t)256 * nb, tmp_keys, tmp_vals);
         std::swap(keys, tmp_keys);
         std::swap(vals, tmp_vals);
     }
