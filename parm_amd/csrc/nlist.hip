@@ -11,7 +11,7 @@
 // Pipeline (one rebuild):
 //   k_cell_keys      key = cell of each member, ncell for non-members
 //   radix passes     k_radix_hist -> k_radix_scan -> k_radix_scatter (8 bits)
-//   gathers          every per-slot array permuted into the new order
+//   k_permute        every per-slot array permuted into the new order
 //   k_cell_bounds    first slot of each cell
 //   k_build_list<D>  full (both directions) list, ELL layout nbr[k*npad + i]
 //
