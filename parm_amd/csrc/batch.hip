@@ -11,7 +11,7 @@
 // Per step:  k_batch_force   forces + kick (+ next drift, |x - xref| per atom)
 //            k_batch_rebuild one workgroup per system; exits unless that
 //                            system's skin flag is set: cell sort in LDS,
-//                            exact list build, xref = x
+//                            sorted position copy, exact list build, xref = x
 //            k_batch_skin    one wavefront per system: top-2 of |x - xref|
 //                            -> the system's flag for the next rebuild
 #include "common.hpp"
@@ -203,37 +203,42 @@ __global__ __launch_bounds__(kBlock) void k_batch_skin(int nsys, BatchArrays A,
 }
 
 // -------------------------------------------------------------- rebuild
-// One workgroup per system.  Exits unless the system's flag is set (or
-// force).  Cell keys and a stable counting sort of the system's atoms by
-// cell in LDS, then the exact list build over the periodic 27-cell
-// stencil (candidates in cell order, then index order: deterministic), then
-// xref = x.  With `alt` (the positions one step ahead), |alt - xref| is
-// recomputed for the next skin test.
+// One workgroup of kRebuildThreads per system.  Exits unless the system's
+// flag is set (or force).  Cell keys and a stable counting sort of the
+// system's atoms by cell in LDS; the sorted positions and diameters are
+// copied to xs / dsort so each candidate is one contiguous load; then the
+// exact list build over the periodic 27-cell stencil (candidates in cell
+// order, then sorted order: deterministic), four candidates in flight per
+// thread; then xref = x.  With `alt` (the positions one step ahead),
+// |alt - xref| is recomputed for the next skin test.
 constexpr int kBatchMaxCells = kBatchMaxCellsPerSystem;
+constexpr int kRebuildThreads = 1024;
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_batch_rebuild(
+__global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
     BatchArrays A, const typename PosStore<D>::T *__restrict__ pos,
     const typename PosStore<D>::T *__restrict__ alt, int force, int count_rebuild,
     double skin, int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
     int32_t *__restrict__ ncount, int32_t *__restrict__ keys,
     int32_t *__restrict__ cellidx, int32_t *__restrict__ cstart,
+    typename PosStore<D>::T *__restrict__ xs, double *__restrict__ dsort,
     uint32_t *__restrict__ flags, uint32_t *__restrict__ which,
     unsigned long long *__restrict__ npairs, uint32_t *__restrict__ gflag) {
     using PT = typename PosStore<D>::T;
+    constexpr int NT = kRebuildThreads;
     const int s = blockIdx.x;
     if (!force && flags[s] == 0u) return;
     __shared__ int hist[kBatchMaxCells];
-    __shared__ int ck[kBlock];
-    __shared__ unsigned long long red[kBlock / kWave];
+    __shared__ int ck[NT];
+    __shared__ unsigned long long red[NT / kWave];
     const BatchGeom G = A.geom[s];
     const int off = G.off, n = G.n, nc0 = G.grid.nc[0], nc1 = G.grid.nc[1],
               nc2 = G.grid.nc[2], ncell = G.grid.ncell;
     const int t = threadIdx.x;
-    for (int c = t; c < ncell; c += kBlock) hist[c] = 0;
+    for (int c = t; c < ncell; c += NT) hist[c] = 0;
     __syncthreads();
     // keys and counts
-    for (int k = t; k < n; k += kBlock) {
+    for (int k = t; k < n; k += NT) {
         const Vec<D> x = PosStore<D>::load(pos[off + k]);
         int key = 0;
 #pragma unroll
@@ -251,8 +256,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
     // exclusive scan of the counts: each thread sums a run of kScanPer
     // cells, a block scan of the run sums, then each thread writes its run
     {
-        constexpr int kScanPer = kBatchMaxCells / kBlock;
-        __shared__ int wsum[kBlock / kWave];
+        constexpr int kScanPer = kBatchMaxCells / NT;
+        __shared__ int wsum[NT / kWave];
         const int c0 = t * kScanPer;
         int loc = 0;
         for (int q = 0; q < kScanPer; q++) loc += (c0 + q < ncell) ? hist[c0 + q] : 0;
@@ -280,8 +285,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
         if (t == 0) cstart[G.cell_off + ncell] = off + n;
     }
     __syncthreads();
-    // stable scatter, one chunk of kBlock atoms at a time
-    for (int base = 0; base < n; base += kBlock) {
+    // stable scatter, one chunk of NT atoms at a time
+    for (int base = 0; base < n; base += NT) {
         const int k = base + t;
         const int key = k < n ? keys[off + k] : -1;
         ck[t] = key;
@@ -295,16 +300,25 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
         if (k < n) atomicAdd(&hist[key], 1);
         __syncthreads();
     }
-    // xref = x (the reference stores lastlocs inside the rebuild loop)
-    for (int k = t; k < n; k += kBlock) static_cast<PT *>(A.xref_w)[off + k] = pos[off + k];
+    // hist[c] is now the end of cell c (system-local sorted index).  Sorted
+    // copies of the positions and diameters, and xref = x (the reference
+    // stores lastlocs inside the rebuild loop).
+    for (int u = t; u < n; u += NT) {
+        const int i = cellidx[off + u];
+        xs[off + u] = pos[i];
+        dsort[off + u] = A.diam[i];
+    }
+    for (int k = t; k < n; k += NT) static_cast<PT *>(A.xref_w)[off + k] = pos[off + k];
     __threadfence_block();
     __syncthreads();
-    // list build
+    // list build: thread per atom in sorted order
     unsigned long long mycnt = 0;
-    for (int k = t; k < n; k += kBlock) {
-        const int i = off + k;
-        const Vec<D> xi = PosStore<D>::load(pos[i]);
-        const double di = A.diam[i];
+    const int lo0 = nc0 >= 3 ? -1 : 0, lo1 = (D >= 2 && nc1 >= 3) ? -1 : 0,
+              lo2 = (D == 3 && nc2 >= 3) ? -1 : 0;
+    for (int u = t; u < n; u += NT) {
+        const int i = cellidx[off + u];
+        const Vec<D> xi = PosStore<D>::load(xs[off + u]);
+        const double di = dsort[off + u];
         int key = keys[i];
         int ci[3] = {0, 0, 0};
         ci[0] = key % nc0;
@@ -314,8 +328,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
             key /= nc1;
         }
         if (D == 3) ci[2] = key % nc2;
-        const int lo0 = nc0 >= 3 ? -1 : 0, lo1 = (D >= 2 && nc1 >= 3) ? -1 : 0,
-                  lo2 = (D == 3 && nc2 >= 3) ? -1 : 0;
         int cnt = 0;
         for (int oz = lo2; oz <= -lo2; oz++)
             for (int oy = lo1; oy <= -lo1; oy++)
@@ -325,32 +337,45 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
                     q1 = q1 < 0 ? q1 + nc1 : (q1 >= nc1 ? q1 - nc1 : q1);
                     q2 = q2 < 0 ? q2 + nc2 : (q2 >= nc2 ? q2 - nc2 : q2);
                     const int cc = (D == 3 ? (q2 * nc1 + q1) * nc0 : q1 * nc0) + q0;
-                    const int s0 = cstart[G.cell_off + cc], s1 = cstart[G.cell_off + cc + 1];
-                    for (int u = s0; u < s1; u++) {
-                        const int j = cellidx[u];
-                        if (j == i) continue;
-                        // minimum image by rint: equal to the reference's
-                        // remainder up to rounding far inside the 1e-6 band,
-                        // where the reference's predicate decides
-                        const Vec<D> xj = PosStore<D>::load(pos[j]);
-                        const double dj = A.diam[j];
-                        double d2 = 0.0;
+                    const int s0 = cc > 0 ? hist[cc - 1] : 0, s1 = hist[cc];
+                    for (int v0 = s0; v0 < s1; v0 += 4) {
+                        // four candidates' loads in flight together
+                        Vec<D> xj[4];
+                        double dj[4];
 #pragma unroll
-                        for (int k = 0; k < D; k++) {
-                            double dk = xi.c[k] - xj.c[k];
-                            dk -= G.box.L[k] * rint(dk * G.box.invL[k]);
-                            d2 = fma(dk, dk, d2);
+                        for (int q = 0; q < 4; q++) {
+                            const int v = v0 + q < s1 ? v0 + q : s0;
+                            xj[q] = PosStore<D>::load(xs[off + v]);
+                            dj[q] = dsort[off + v];
                         }
-                        const double cc = (di + dj) / 2 + skin;
-                        const double c2 = cc * cc;
-                        const bool pass =
-                            d2 < c2 * (1.0 - 1e-6) ||
-                            (d2 <= c2 * (1.0 + 1e-6) && in_list<D>(xi, xj, di, dj, G.box, skin));
-                        if (pass) {
-                            if (cnt < cap)
-                                nbr[untiled_entry(cnt, i, tpa, npad)] =
-                                    ((uint32_t)j << type_bits) | (uint32_t)A.type[j];
-                            cnt++;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int v = v0 + q;
+                            if (v >= s1 || v == u) continue;
+                            // minimum image by rint: equal to the reference's
+                            // remainder up to rounding far inside the 1e-6
+                            // band, where the reference's predicate decides
+                            double d2 = 0.0;
+#pragma unroll
+                            for (int k = 0; k < D; k++) {
+                                double dk = xi.c[k] - xj[q].c[k];
+                                dk -= G.box.L[k] * rint(dk * G.box.invL[k]);
+                                d2 = fma(dk, dk, d2);
+                            }
+                            const double cut = (di + dj[q]) / 2 + skin;
+                            const double c2 = cut * cut;
+                            const bool pass =
+                                d2 < c2 * (1.0 - 1e-6) ||
+                                (d2 <= c2 * (1.0 + 1e-6) &&
+                                 in_list<D>(xi, xj[q], di, dj[q], G.box, skin));
+                            if (pass) {
+                                if (cnt < cap) {
+                                    const int j = cellidx[off + v];
+                                    nbr[untiled_entry(cnt, i, tpa, npad)] =
+                                        ((uint32_t)j << type_bits) | (uint32_t)A.type[j];
+                                }
+                                cnt++;
+                            }
                         }
                     }
                 }
@@ -372,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_rebuild(
     __syncthreads();
     if (t == 0) {
         unsigned long long tot = 0;
-        for (int q = 0; q < kBlock / kWave; q++) tot += red[q];
+        for (int q = 0; q < NT / kWave; q++) tot += red[q];
         npairs[s] = tot / 2;
         if (count_rebuild) which[s] += 1u;
         flags[s] = 0u;
@@ -546,20 +571,20 @@ hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const 
                                 const void *alt, int force, int count_rebuild, double skin,
                                 int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                                 int32_t *ncount, int32_t *keys, int32_t *cellidx,
-                                int32_t *cstart, uint32_t *flags, uint32_t *which,
-                                unsigned long long *npairs, uint32_t *gflag,
+                                int32_t *cstart, void *xs, double *dsort, uint32_t *flags,
+                                uint32_t *which, unsigned long long *npairs, uint32_t *gflag,
                                 hipStream_t s) {
     if (nsys <= 0) return hipSuccess;
     if (ndim == 3)
-        hipLaunchKernelGGL(k_batch_rebuild<3>, dim3(nsys), dim3(kBlock), 0, s, A,
+        hipLaunchKernelGGL(k_batch_rebuild<3>, dim3(nsys), dim3(kRebuildThreads), 0, s, A,
                            (const double4 *)pos, (const double4 *)alt, force, count_rebuild,
                            skin, type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx,
-                           cstart, flags, which, npairs, gflag);
+                           cstart, (double4 *)xs, dsort, flags, which, npairs, gflag);
     else
-        hipLaunchKernelGGL(k_batch_rebuild<2>, dim3(nsys), dim3(kBlock), 0, s, A,
+        hipLaunchKernelGGL(k_batch_rebuild<2>, dim3(nsys), dim3(kRebuildThreads), 0, s, A,
                            (const double2 *)pos, (const double2 *)alt, force, count_rebuild,
                            skin, type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx,
-                           cstart, flags, which, npairs, gflag);
+                           cstart, (double2 *)xs, dsort, flags, which, npairs, gflag);
     return hipGetLastError();
 }
 

@@ -82,6 +82,8 @@ struct parm_batch {
     bool tuned = false, tpa_fixed = false, built = false;
     uint32_t *nbr = nullptr;
     int32_t *ncount = nullptr, *keys = nullptr, *cellidx = nullptr, *cstart = nullptr;
+    void *xs = nullptr;        // positions in each system's cell order (rebuild scratch)
+    double *dsort = nullptr;   // list diameters in the same order
     uint32_t *flags = nullptr, *which = nullptr, *gflag = nullptr, *hgflag = nullptr;
     unsigned long long *npairs = nullptr;
 
@@ -163,6 +165,8 @@ void bdestroy(parm_batch *b) {
     bfree(b->ncount);
     bfree(b->keys);
     bfree(b->cellidx);
+    bfree(b->xs);
+    bfree(b->dsort);
     bfree(b->cstart);
     bfree(b->flags);
     bfree(b->which);
@@ -262,7 +266,8 @@ int brebuild(parm_batch *b, bool force, bool count, const void *pos,
     PB_HIP(launch_batch_rebuild(b->ndim, b->nsys, b->arrays(), pos, alt, force ? 1 : 0,
                                 count ? 1 : 0, b->skin, b->type_bits, b->tpa, b->cap,
                                 b->npad, b->nbr, b->ncount, b->keys, b->cellidx, b->cstart,
-                                b->flags, b->which, b->npairs, b->gflag, b->stream));
+                                b->xs, b->dsort, b->flags, b->which, b->npairs, b->gflag,
+                                b->stream));
     return PARM_OK;
 }
 
@@ -488,6 +493,7 @@ int parm_batch_create(int ndim, int precision, int64_t nsys, const int64_t *nato
         (rc = balloc(&b->stage, n * ndim)) || (rc = balloc(&b->type, n)) ||
         (rc = balloc(&b->sys, n)) || (rc = balloc(&b->ncount, (size_t)b->npad)) ||
         (rc = balloc(&b->keys, n)) || (rc = balloc(&b->cellidx, n)) ||
+        (rc = balloc((char **)&b->xs, n * b->pos_bytes)) || (rc = balloc(&b->dsort, n)) ||
         (rc = balloc(&b->flags, (size_t)nsys)) || (rc = balloc(&b->which, (size_t)nsys)) ||
         (rc = balloc(&b->gflag, 1)) || (rc = balloc(&b->npairs, (size_t)nsys)) ||
         (rc = balloc(&b->eatom, 2 * n)) || (rc = balloc(&b->Usys, (size_t)nsys)) ||

@@ -250,8 +250,9 @@ hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const 
                                 const void *alt, int force, int count_rebuild, double skin,
                                 int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                                 int32_t *ncount, int32_t *keys, int32_t *cellidx,
-                                int32_t *cstart, uint32_t *flags, uint32_t *which,
-                                unsigned long long *npairs, uint32_t *gflag, hipStream_t s);
+                                int32_t *cstart, void *xs, double *dsort, uint32_t *flags,
+                                uint32_t *which, unsigned long long *npairs, uint32_t *gflag,
+                                hipStream_t s);
 hipError_t launch_batch_sys_sums(int nsys, const BatchArrays &A, const double *eatom,
                                  double *U, double *contacts, hipStream_t s);
 constexpr int kBatchMaxCellsPerSystem = 4096;
