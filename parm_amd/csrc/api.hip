@@ -345,6 +345,13 @@ int launch_tile_plan(parm_system *s) {
     return PARM_OK;
 }
 
+// The tiled force kernel's staged stencil fits the LDS it may take (the
+// list build's staging is smaller: a plan the build accepts can still need
+// the untiled force layout).
+bool tiled_force_fits(const parm_system *s) {
+    return tiled_force_lds(s->prec, s->pair_kind, s->smax) <= s->lds_max;
+}
+
 void accept_tiles(parm_system *s, int nt, int smax, int bad) {
     s->tiled = false;
     s->tile_ok = false;
@@ -354,9 +361,9 @@ void accept_tiles(parm_system *s, int nt, int smax, int bad) {
     if (s->n == 0 || bad || nt <= 0 || nt > pstride || tiled_build_lds(smax) > s->lds_max)
         return;
     s->tile_ok = true;
-    s->tiled = s->layout_pref != 2;
     s->ntiles = nt;
     s->smax = std::max(smax, 1);
+    s->tiled = s->layout_pref != 2 && tiled_force_fits(s);
 }
 
 // The plan queued by launch_tile_plan, read back (one host round trip).
@@ -440,7 +447,9 @@ int build_speculative(parm_system *s) {
     const int grid = std::min(s->tiles_cap, pstride);
     const int limit = (int)(s->lds_max / tiled_build_lds(1));
     const int bound = std::min(limit, s->spec_smax + std::max(s->spec_smax / 8, 64));
-    const bool tiled = s->layout_pref != 2;
+    // the layout the force kernel last took (accept_tiles re-decides it
+    // from the plan as read; a mismatch rebuilds)
+    const bool tiled = s->tiled;
     PARM_HIP(launch_build_tiled(s->ndim, s->n, grid, bound, s->tiles, s->truns, s->pos, s->xwrap,
                                 s->rank,
                                 s->diam, s->type, s->type_bits, s->keys, s->cstart, s->grid,
@@ -573,6 +582,7 @@ struct ProfScope {
 // with the cell keys and bounds of that rebuild.
 int relayout(parm_system *s) {
     if (!s->built || s->n == 0) return PARM_OK;
+    if (s->tiled && !(s->tile_ok && tiled_force_fits(s))) s->tiled = false;
     PARM_CHECK(build_checked(s, s->xref));
     return PARM_OK;
 }
@@ -588,8 +598,9 @@ int autotune(parm_system *s, int stage, const StepCoef &c) {
     if (s->tuned || !s->built || s->n == 0) return PARM_OK;
     std::vector<std::pair<int, int>> cands;  // (tiled, tpa)
     std::vector<int> layouts;
-    if (s->tile_ok && s->layout_pref != 2) layouts.push_back(1);
-    if (s->layout_pref != 1 || !s->tile_ok) layouts.push_back(0);
+    const bool tiled_ok = s->tile_ok && tiled_force_fits(s);
+    if (tiled_ok && s->layout_pref != 2) layouts.push_back(1);
+    if (s->layout_pref != 1 || !tiled_ok) layouts.push_back(0);
     for (int lay : layouts) {
         if (s->tpa_fixed) {
             cands.emplace_back(lay, s->tpa);

@@ -186,6 +186,9 @@ hipError_t launch_force(int ndim, int precision, int stage, bool energy,
                         const AtomArrays &A, const ListView &Lv, const PairView &P,
                         const Box &box, const StepCoef &c, const ReduceBuf &R,
                         hipStream_t s);
+// Dynamic LDS of the tiled force kernel for a stencil of smax staged slots
+// (precision, pair kind; any other kind gets the largest staging).
+size_t tiled_force_lds(int prec, int kind, int smax);
 // Tiled force launch (step_tiled.hip); same contract as launch_force.
 hipError_t launch_force_tiled(int ndim, int precision, int stage, bool energy,
                               const AtomArrays &A, const ListView &Lv, const PairView &P,

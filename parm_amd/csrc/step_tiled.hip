@@ -298,14 +298,14 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
 }
 
 // ======================================================== host launchers
-namespace {
-
 size_t tiled_force_lds(int prec, int kind, int smax) {
-    if (prec == kFP64 && kind != kPairLJ) return (size_t)smax * sizeof(double4);
+    if (prec == kFP64 && kind == kPairRepulsion) return (size_t)smax * sizeof(double4);
     if (prec != kFP64 && kind == kPairLJ)
         return (size_t)smax * (sizeof(float4) + sizeof(int32_t));
     return (size_t)smax * (sizeof(double4) + sizeof(int32_t));
 }
+
+namespace {
 
 template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, int TPA>
 void tiled_tt(const AtomArrays &A, const ListView &Lv, const PairView &P, const Box &box,

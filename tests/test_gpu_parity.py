@@ -370,3 +370,30 @@ def test_every_lanes_per_atom_layout(tpa, monkeypatch):
     f_dev, l_dev = s.pairs()
     assert np.array_equal(f_dev, first) and np.array_equal(l_dev, last)
     assert abs(s.pair_sums()["U"] - ref["U"]) <= 1e-12 * abs(ref["U"])
+
+
+def test_rebuild_after_denser_packing_replans():
+    """A rebuild whose largest tile stencil outgrows the previous plan by more
+    than the speculative build's margin (the atoms squeezed into 3/4 of the
+    box along y: a third more atoms per stencil row), so the build first
+    queued against the old bound is redone from the plan as read; then the
+    way back (a bound larger than needed).  Pair set and FP64 forces match
+    the oracle each time."""
+    _device_or_skip()
+    n = 32768
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=11)
+    p = S.lj_params(n)
+    t = _table(O.LJ_ATTRACT_REPULSE, p)
+    s = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3)
+    squeezed = x.copy()
+    squeezed[:, 1] = (squeezed[:, 1] + L[1] / 2) * 0.75 - L[1] / 2
+    for step, xs in enumerate([x, squeezed, x]):
+        print(f"replan case step {step}", flush=True)
+        s.set_state(x=xs)
+        assert s.update_list(True)
+        first, last = O.nlist(xs, L, p["diam"], 0.3)
+        f_dev, l_dev = s.pairs()
+        assert np.array_equal(f_dev, first) and np.array_equal(l_dev, last)
+        s.set_forces(False)
+        ref = O.forces(xs, L, t, first, last)
+        assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
