@@ -349,7 +349,8 @@ int launch_tile_plan(parm_system *s) {
 // list build's staging is smaller: a plan the build accepts can still need
 // the untiled force layout).
 bool tiled_force_fits(const parm_system *s) {
-    return tiled_force_lds(s->prec, s->pair_kind, s->smax) <= s->lds_max;
+    return tiled_force_lds(s->ndim, s->prec, s->pair_kind, s->single != 0, s->smax) <=
+           s->lds_max;
 }
 
 void accept_tiles(parm_system *s, int nt, int smax, int bad) {

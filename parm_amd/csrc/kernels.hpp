@@ -187,8 +187,9 @@ hipError_t launch_force(int ndim, int precision, int stage, bool energy,
                         const Box &box, const StepCoef &c, const ReduceBuf &R,
                         hipStream_t s);
 // Dynamic LDS of the tiled force kernel for a stencil of smax staged slots
-// (precision, pair kind; any other kind gets the largest staging).
-size_t tiled_force_lds(int prec, int kind, int smax);
+// (dimension, precision, pair kind, one pair type or several; no pair kind
+// yet gets the largest staging).
+size_t tiled_force_lds(int ndim, int prec, int kind, bool single, int smax);
 // Tiled force launch (step_tiled.hip); same contract as launch_force.
 hipError_t launch_force_tiled(int ndim, int precision, int stage, bool energy,
                               const AtomArrays &A, const ListView &Lv, const PairView &P,

@@ -45,7 +45,11 @@ namespace parm {
 template <int D, bool ENERGY, typename Exact>
 __device__ __forceinline__ void lj64x(const PairParam &pp, const Vec<D> &r,
                                       const Exact &exact_d2, double (&f)[D], PairAcc &acc) {
-    const double d2 = sqnorm<D>(r);
+    // fused d2: within an ulp of the reference's (x*x + y*y) + z*z, far
+    // inside the band in which exact_d2 decides
+    double d2 = r.c[0] * r.c[0];
+#pragma unroll
+    for (int k = 1; k < D; k++) d2 = fma(r.c[k], r.c[k], d2);
     if (!(d2 < pp.lim_lo)) {
         if (d2 > pp.lim_hi) return;
         if (exact_d2() / pp.sig2 > pp.cut2) return;
@@ -93,11 +97,18 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     constexpr bool ENERGY = STAGE == kStageSums;
     constexpr bool RAW = PREC == kFP64 && KIND != kPairLJ;
     constexpr bool REL32 = PREC != kFP64 && KIND == kPairLJ;
+    // COMPACT (one pair type, offsets staged): a slot is the D offsets only
+    // (FP64: 8 D bytes, FP32: 4 D bytes), no type, no atom index (the rare
+    // exact-d2 path finds the atom from the runs): less LDS per workgroup,
+    // more workgroups per CU.
+    constexpr bool COMPACT = SINGLE && !RAW;
     constexpr int NT = kWave * TPA;
     extern __shared__ double4 st64[];
     __shared__ int2 runs[kTileRuns];
     __shared__ int4 hdr;
     float4 *st32 = reinterpret_cast<float4 *>(st64);
+    double *stc64 = reinterpret_cast<double *>(st64);
+    float *stc32 = reinterpret_cast<float *>(st64);
     int32_t *stj = REL32 ? reinterpret_cast<int32_t *>(st32 + Lv.smax)
                          : reinterpret_cast<int32_t *>(st64 + Lv.smax);
 
@@ -132,9 +143,17 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
             v = offset_from<D>(xj, xo, box);
 #pragma unroll
             for (int k = 0; k < D; k++) wide |= fabs(v.c[k]) >= 0.5 * box.halfL[k];
-            stj[s] = j;
+            if (!COMPACT) stj[s] = j;
         }
-        if (REL32) {
+        if (COMPACT) {
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                if (REL32)
+                    stc32[s * D + k] = (float)v.c[k];
+                else
+                    stc64[s * D + k] = v.c[k];
+            }
+        } else if (REL32) {
             float4 q;
             q.x = (float)v.c[0];
             q.y = (float)v.c[1];
@@ -198,7 +217,14 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
         }
         // the reference's d2 (for in/out decisions at the cutoff)
         auto exact_d2 = [&](int s) {
-            const Vec<D> r = diff<D>(xi, PosStore<D>::load(pos[stj[s]]), box);
+            int j;
+            if constexpr (COMPACT) {
+                const int r = slot_run(runs, s);
+                j = runs[r].x + (s - runs[r].y);
+            } else {
+                j = stj[s];
+            }
+            const Vec<D> r = diff<D>(xi, PosStore<D>::load(pos[j]), box);
             return sqnorm<D>(r);
         };
         auto pair = [&](int s, auto fold_c) {
@@ -213,8 +239,18 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                 const Vec<D> r = diff<D>(xi, xj, box);
                 visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : prow[tj], r, fi, f32, acc);
             } else if constexpr (REL32) {
-                const float4 q = st32[s];
-                const float qc[3] = {q.x, q.y, q.z};
+                float qc[3] = {0.0f, 0.0f, 0.0f};
+                int tj = 0;
+                if constexpr (COMPACT) {
+#pragma unroll
+                    for (int k = 0; k < D; k++) qc[k] = stc32[s * D + k];
+                } else {
+                    const float4 q = st32[s];
+                    qc[0] = q.x;
+                    qc[1] = q.y;
+                    qc[2] = q.z;
+                    tj = __float_as_int(q.w);
+                }
                 float rf[D];
 #pragma unroll
                 for (int k = 0; k < D; k++) {
@@ -222,12 +258,21 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                     if (FOLD) d = d > hLf[k] ? d - Lf[k] : (d < -hLf[k] ? d + Lf[k] : d);
                     rf[k] = d;
                 }
-                const int tj = SINGLE ? 0 : __float_as_int(q.w);
                 lj32f<D, ENERGY>(SINGLE ? p0 : prow[tj], rf, [&]() { return exact_d2(s); },
                                  f32, acc);
             } else {
-                const double4 q = st64[s];
-                const double qc[3] = {q.x, q.y, q.z};
+                double qc[3] = {0.0, 0.0, 0.0};
+                int tj = 0;
+                if constexpr (COMPACT) {
+#pragma unroll
+                    for (int k = 0; k < D; k++) qc[k] = stc64[s * D + k];
+                } else {
+                    const double4 q = st64[s];
+                    qc[0] = q.x;
+                    qc[1] = q.y;
+                    qc[2] = q.z;
+                    tj = (int)(D == 3 ? q.w : q.z);
+                }
                 Vec<D> r;
 #pragma unroll
                 for (int k = 0; k < D; k++) {
@@ -237,7 +282,6 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                                              : (d < -box.halfL[k] ? d + box.L[k] : d);
                     r.c[k] = d;
                 }
-                const int tj = SINGLE ? 0 : (int)(D == 3 ? q.w : q.z);
                 const PairParam &pp = SINGLE ? p0 : prow[tj];
                 if constexpr (PREC == kFP64) {
                     lj64x<D, ENERGY>(pp, r, [&]() { return exact_d2(s); }, fi, acc);
@@ -249,13 +293,15 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                 }
             }
         };
-        // two lane entries per 32-bit word; the next word is loaded while
-        // the current one is evaluated
+        // two lane entries per 32-bit word; the list streams from memory,
+        // so the word two ahead is loaded while the current one is evaluated
         auto loop = [&](auto fold_c) {
-            uint32_t wv = nlane > 0 ? wp[0] : 0u;
+            uint32_t w0 = nlane > 0 ? wp[0] : 0u;
+            uint32_t w1 = nlane > 2 ? wp[wstride] : 0u;
             for (int m = 0; m < nlane; m += 2) {
-                const uint32_t cur = wv;
-                if (m + 2 < nlane) wv = wp[(size_t)((m >> 1) + 1) * wstride];
+                const uint32_t cur = w0;
+                w0 = w1;
+                if (m + 4 < nlane) w1 = wp[(size_t)((m >> 1) + 2) * wstride];
                 pair((int)(cur & 0xffffu), fold_c);
                 if (m + 1 < nlane) pair((int)(cur >> 16), fold_c);
             }
@@ -294,14 +340,17 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
             atom_store<D, STAGE>(A, a0 + (int)threadIdx.x, fa, ain, c, top, nonfinite);
         }
     }
-    group_partials<D, STAGE, ENERGY, TPA>(acc, top, nonfinite, R);
+    group_partials<D, STAGE, ENERGY, TPA, true>(acc, top, nonfinite, R);
 }
 
 // ======================================================== host launchers
-size_t tiled_force_lds(int prec, int kind, int smax) {
-    if (prec == kFP64 && kind == kPairRepulsion) return (size_t)smax * sizeof(double4);
-    if (prec != kFP64 && kind == kPairLJ)
-        return (size_t)smax * (sizeof(float4) + sizeof(int32_t));
+size_t tiled_force_lds(int ndim, int prec, int kind, bool single, int smax) {
+    const bool raw = prec == kFP64 && kind == kPairRepulsion;
+    const bool rel32 = prec != kFP64 && kind == kPairLJ;
+    if (raw) return (size_t)smax * sizeof(double4);
+    if (single && kind != kPairNone)
+        return (size_t)smax * ndim * (rel32 ? sizeof(float) : sizeof(double));
+    if (rel32) return (size_t)smax * (sizeof(float4) + sizeof(int32_t));
     return (size_t)smax * (sizeof(double4) + sizeof(int32_t));
 }
 
@@ -313,7 +362,7 @@ void tiled_tt(const AtomArrays &A, const ListView &Lv, const PairView &P, const 
     constexpr bool DRIFT =
         (STAGE == kStageVerletKickDrift || STAGE == kStageDampedKickDrift);
     constexpr bool ENERGY = STAGE == kStageSums;
-    const size_t lds = tiled_force_lds(PREC, KIND, Lv.smax);
+    const size_t lds = tiled_force_lds(D, PREC, KIND, SINGLE, Lv.smax);
     hipLaunchKernelGGL((k_force_tiled<D, KIND, PREC, SINGLE, HARM, STAGE, TPA>),
                        dim3(Lv.ntiles), dim3(kWave * TPA), lds, s, A, Lv, P, box, c, R);
     if (ENERGY || DRIFT) (void)launch_finalize_rt(ENERGY, DRIFT, Lv.ntiles, R, s);

@@ -38,7 +38,9 @@ __device__ __forceinline__ uint32_t skin_flag(const Top2 &tt, double nft, double
     return fl;
 }
 
-template <bool SUMS, bool SKIN, int NW>
+// SKIN_W0: only the first wavefront holds skin-test values (the others'
+// are the neutral ones and skip their wavefront reduction).
+template <bool SUMS, bool SKIN, int NW, bool SKIN_W0 = false>
 __device__ __forceinline__ void block_partial(double (&v)[kRedWidth], Top2 t,
                                               const ReduceBuf &R) {
     __shared__ double sh[NW][kRedWidth];
@@ -47,7 +49,7 @@ __device__ __forceinline__ void block_partial(double (&v)[kRedWidth], Top2 t,
 #pragma unroll
         for (int k = 0; k < kTop1; k++) v[k] = wave_sum(v[k]);
     }
-    if (SKIN) {
+    if (SKIN && (!SKIN_W0 || w == 0)) {
         t = wave_top2(t);
         v[kNonFinite] = wave_sum(v[kNonFinite]);
     }
@@ -190,8 +192,9 @@ __device__ __forceinline__ void atom_apply(const AtomArrays &A, int i,
     atom_store<D, STAGE>(A, i, fi, in, c, top, nonfinite);
 }
 
-// This workgroup's reduction partials (all threads call it).
-template <int D, int STAGE, bool ENERGY, int NW>
+// This workgroup's reduction partials (all threads call it).  SKIN_W0: the
+// stage's writes (and so the skin test) were done by the first wavefront.
+template <int D, int STAGE, bool ENERGY, int NW, bool SKIN_W0 = false>
 __device__ __forceinline__ void group_partials(const PairAcc &acc, Top2 top, double nonfinite,
                                                const ReduceBuf &R) {
     constexpr bool DRIFT =
@@ -209,7 +212,7 @@ __device__ __forceinline__ void group_partials(const PairAcc &acc, Top2 top, dou
             v[kSumOverlaps] = acc.overlaps;
         }
         v[kNonFinite] = nonfinite;
-        block_partial<ENERGY, DRIFT, NW>(v, top, R);
+        block_partial<ENERGY, DRIFT, NW, SKIN_W0>(v, top, R);
     }
 }
 
