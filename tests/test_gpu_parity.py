@@ -243,21 +243,40 @@ def test_list_update_order_with_zero_skin(integrator):
     assert dev.which() == 1 + 24  # the initial build + one per step
 
 
-@pytest.mark.parametrize("kind,ndim,n,rho,skin,bi", [
-    (O.LJ_ATTRACT_REPULSE, 3, 30000, 0.8, 0.3, False),   # row tiles, normal stencils
-    (O.LJ_ATTRACT_REPULSE, 3, 4096, 0.8, 0.3, False),    # full-row stencils
-    (O.REPULSION, 3, 20000, 0.65, 0.2, True),            # few atoms per cell
-    (O.REPULSION, 2, 20000, 1.0, 0.2, True),
+def _tile_case_params(pset, n):
+    if pset == "lj":
+        return S.lj_params(n)
+    if pset == "bi":
+        return S.bidisperse_repulsion(n, n)
+    if pset == "mono":  # one repulsion type
+        return dict(eps=[1.0], sigma=[1.2], third=[2.0], types=np.zeros(n, np.int32),
+                    diam=np.full(n, 1.2))
+    # two LJ types
+    rng = np.random.default_rng(n)
+    types = (rng.random(n) < 0.3).astype(np.int32)
+    sigma = np.array([1.0, 0.9])
+    return dict(eps=[[1.0, 0.8], [0.8, 0.5]], sigma=list(sigma), third=[2.5, 2.5],
+                types=types, diam=sigma[types] * 2.5)
+
+
+@pytest.mark.parametrize("kind,ndim,n,rho,skin,pset", [
+    (O.LJ_ATTRACT_REPULSE, 3, 30000, 0.8, 0.3, "lj"),    # row tiles, normal stencils
+    (O.LJ_ATTRACT_REPULSE, 3, 4096, 0.8, 0.3, "lj"),     # full-row stencils
+    (O.REPULSION, 3, 20000, 0.65, 0.2, "bi"),            # few atoms per cell
+    (O.REPULSION, 2, 20000, 1.0, 0.2, "bi"),
+    (O.REPULSION, 3, 20000, 0.65, 0.2, "mono"),          # one type: compact staging
+    (O.LJ_ATTRACT_REPULSE, 3, 20000, 0.8, 0.3, "lj2"),   # two types: typed slots
 ])
 @pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED])
-def test_tiled_layout_matches_untiled(kind, ndim, n, rho, skin, bi, prec, monkeypatch):
+def test_tiled_layout_matches_untiled(kind, ndim, n, rho, skin, pset, prec, monkeypatch):
     """The LDS-tiled list and force kernels against the untiled ones on the
     same state: identical pair sets, forces within summation-order rounding
     (FP64) or the mixed bar, and identical trajectories' rebuild counts."""
     _device_or_skip()
-    print(f"tiled vs untiled kind={kind} ndim={ndim} n={n} prec={prec}", flush=True)
+    print(f"tiled vs untiled kind={kind} ndim={ndim} n={n} pset={pset} prec={prec}",
+          flush=True)
     x, L = S.jittered_lattice(n, rho, ndim, seed=n + 5)
-    p = S.bidisperse_repulsion(n, n) if bi else S.lj_params(n)
+    p = _tile_case_params(pset, n)
     v = S.gaussian_velocities(n, ndim, 1.0, seed=9)
     sys_ = {}
     for tiled in (1, 0):
