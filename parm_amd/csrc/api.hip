@@ -292,8 +292,10 @@ int initial_capacity(const parm_system *s) {
 // a fused step loop; xref is free to take one of them, the list build
 // rewrites it), the ndim components of v, a, f, the masses and diameters,
 // and the type / original index / member rank.  Each lands in its spare,
-// which then becomes the column.
-int permute_all(parm_system *s, const int32_t *perm) {
+// which then becomes the column.  keep_af = false leaves a and f in the old
+// order: for a rebuild inside the step loop whose next launch writes both
+// before anything reads them.
+int permute_all(parm_system *s, const int32_t *perm, bool keep_af = true) {
     PermuteSet P{};
     P.pos_src[0] = s->pos;
     P.pos_dst[0] = s->xref;
@@ -301,8 +303,11 @@ int permute_all(parm_system *s, const int32_t *perm) {
     P.pos_dst[1] = s->pos_spare;
     double **dcols[kPermD];
     int nd = 0;
-    for (double **vec : {s->v, s->a, s->f})
-        for (int k = 0; k < s->ndim; k++) dcols[nd++] = &vec[k];
+    for (int k = 0; k < s->ndim; k++) dcols[nd++] = &s->v[k];
+    if (keep_af) {
+        for (double **vec : {s->a, s->f})
+            for (int k = 0; k < s->ndim; k++) dcols[nd++] = &vec[k];
+    }
     dcols[nd++] = &s->m;
     dcols[nd++] = &s->diam;
     for (int c = 0; c < nd; c++) {
@@ -474,7 +479,7 @@ int build_speculative(parm_system *s) {
 
 // NeighborList::update_list's rebuild branch (src/trackers.cpp:55-69):
 // lastlocs = x, then the full pair list from the predicate.
-int rebuild(parm_system *s) {
+int rebuild(parm_system *s, bool keep_af = true) {
     if (!s->list_cfg) return fail(PARM_E_STATE, "neighbour list not configured");
     hipStream_t st = s->stream;
     const int n = s->n;
@@ -492,7 +497,7 @@ int rebuild(parm_system *s) {
         int bits = 1;
         while ((1u << bits) <= (uint32_t)s->grid.ncell && bits < 32) bits++;
         PARM_HIP(radix_sort(n, bits, s->keys, s->vals, s->tkeys, s->tvals, s->hist, st));
-        PARM_CHECK(permute_all(s, s->vals));
+        PARM_CHECK(permute_all(s, s->vals, keep_af));
     }
     PARM_HIP(launch_cell_bounds(n, s->grid.ncell, s->keys, s->cstart, st));
     if (s->cap == 0) PARM_CHECK(ensure_list_capacity(s, initial_capacity(s)));
@@ -732,7 +737,9 @@ int run_steps(parm_system *s, bool damped, const StepCoef &c, int64_t nsteps) {
         if ((fl & kFlagRebuild) || s->ignorechanged) {
             {
                 ProfScope ps(s, &s->prof_rebuild);
-                PARM_CHECK(rebuild(s));
+                // before the last step's end the next launch rewrites a and
+                // f before anything reads them: they need no permutation
+                PARM_CHECK(rebuild(s, last));
             }
             if (!last) {
                 AtomArrays A = s->arrays();

@@ -74,7 +74,9 @@ __global__ __launch_bounds__(kFinal) void k_finalize(int nparts, ReduceBuf R) {
     if constexpr (SKIN) {
         Top2 bt;
         double nf = 0.0;
-#pragma unroll 4
+        // deep unroll: a partial array of ~10^4 workgroups is read in one or
+        // two rounds of loads in flight
+#pragma unroll 16
         for (int b = tid; b < nparts; b += kFinal) {
             Top2 o;
             o.a = R.part[kTop1 * ps + b];
