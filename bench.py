@@ -55,8 +55,8 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE by 1.0, the factors that make the same counters read 1 GiB for a
 # 1 GiB device copy (profiles/k_force_lj_1M_pmc.txt).
 TRAFFIC_BYTES_PER_ATOM = {  # (tiled list, lanes per atom) -> bytes per atom
-    (True, 1): 451.3, (True, 2): 450.6, (True, 4): 454.1, (True, 8): 464.8,
-    (False, 1): 603.8, (False, 2): 614.0, (False, 4): 615.4, (False, 8): 619.9,
+    (True, 1): 451.4, (True, 2): 450.8, (True, 4): 454.2, (True, 8): 465.3,
+    (False, 1): 603.6, (False, 2): 612.9, (False, 4): 615.3, (False, 8): 619.9,
 }
 
 
