@@ -1,6 +1,6 @@
 """Device path vs the CPU oracle, through the C ABI (libparm_hip.so).
 
-Bars (north-star and SURVEY parity rules):
+Bars:
   * neighbour-pair set: bit-identical (same pairs, same reference order);
   * FP64 forces: |F_dev - F_ref| <= 1e-12 * sum_j |f_ij| per atom;
   * FP64 energy / virial / stress: 1e-12 relative; contacts / overlaps exact;
