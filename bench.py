@@ -359,6 +359,78 @@ def relax2d(args):
     return out
 
 
+def lj1000(args):
+    """The reference's CPU-runnable LJ case: N = 1000, density 0.8, T = 1,
+    cutoff 2.5, skin 0.4, velocity Verlet dt = 0.001, 1000 steps, FP64,
+    rebuilds included.  The oracle (the restatement of ParM's single-core
+    path, with its O(N^2) rebuild) runs the same 1000 steps from the same state
+    for the CPU figure, measured in full; the two final states are compared."""
+    from oracle import oracle as O
+
+    n, steps, dt, skin = 1000, 1000, 0.001, 0.4
+    x, v, L = workloads.lj_liquid(n, rho=0.8, T=1.0, min_sep=0.8, seed=args.seed, log=log)
+    # untimed warm-up system (autotuner, first allocations)
+    w = make_lj(x, v, L, capi.FP64, skin)
+    w.verlet(dt, 64)
+    w.synchronize()
+    w.close()
+    s = make_lj(x, v, L, capi.FP64, skin)
+    E0 = total_energy(s)
+    w0 = s.which()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.verlet(dt, steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    rebuilds = s.which() - w0
+    xg = s.positions()
+    dE = abs(total_energy(s) - E0) / abs(E0)
+    s.close()
+    out = {
+        "metric": "atom_timesteps_per_s",
+        "value": n * steps / wall,
+        "unit": "atom-timesteps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform positions pushed apart to 0.8 sigma, Gaussian "
+                "velocities at T=1)",
+        "config": {"workload": "3D LJ (LJAttractRepulsePair) NVE velocity Verlet, plain FP64, "
+                               "the reference's CPU-runnable case",
+                   "natoms": n, "density": 0.8, "cutoff": 2.5, "skin": skin, "dt": dt,
+                   "T0": 1.0, "seed": args.seed},
+        "rebuilds": rebuilds,
+        "energy_drift_rel": dE,
+    }
+    if not args.no_cpu:
+        log("bench: lj1000 cpu baseline (full run) ...")
+        t = O.PairTable(O.LJ_ATTRACT_REPULSE, [[1.0]], [1.0], [2.5], np.zeros(n, np.int32))
+        c = O.System(np.ones(n), L, t, np.full(n, 2.5), skin, ndim=3, brute=True)
+        c.set_state(x=x, v=v, a=np.zeros_like(x))
+        c.initialize()
+        cw0 = c.which()
+        t0 = time.perf_counter()
+        c.verlet(dt, steps)
+        cwall = time.perf_counter() - t0
+        xc = c.state()[0]
+        out["cpu_baseline"] = {
+            "value": n * steps / cwall,
+            "unit": "atom-timesteps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"oracle restatement of ParM's CPU path, 1 core: the full {steps}-step "
+                      f"run with its O(N^2) rebuilds, measured ({cwall:.2f} s)",
+        }
+        out["parity"] = {
+            "rebuilds_gpu": rebuilds,
+            "rebuilds_cpu": c.which() - cw0,
+            "max_abs_dx_final": float(np.max(np.abs(xg - xc))),
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -379,8 +451,9 @@ def main():
     ap.add_argument("--batch-skin", type=float, default=0.2)
     ap.add_argument("--batch-precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--batch-seq-sample", type=int, default=32)
-    ap.add_argument("--workload", choices=["lj", "relax2d"], default="lj",
-                    help="relax2d: the 2D packing relaxation alone (its own JSON line)")
+    ap.add_argument("--workload", choices=["lj", "relax2d", "lj1000"], default="lj",
+                    help="relax2d: the 2D packing relaxation; lj1000: the 1000-atom LJ "
+                         "case beside the oracle's full CPU run (each its own JSON line)")
     ap.add_argument("--relax-natoms", type=int, default=262144)
     ap.add_argument("--relax-precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--relax-dt", type=float, default=0.3)
@@ -401,6 +474,9 @@ def main():
         sys.exit(1)
     if args.workload == "relax2d":
         print(json.dumps(relax2d(args)), flush=True)
+        return
+    if args.workload == "lj1000":
+        print(json.dumps(lj1000(args)), flush=True)
         return
 
     n = args.natoms
