@@ -1100,7 +1100,10 @@ hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys
 }
 
 constexpr int kTiledLanes = 4;   // lanes per atom in k_build_tiled
-constexpr int kTiledUnroll = 8;
+// candidates per lane per round: a stencil row (3 cells) of a dense system
+// (the LJ liquid: ~50 atoms) fills two rounds of 4 x 8; a sparse one (packed
+// discs / spheres: ~7-9 atoms) one round of 4 x 2
+constexpr int kTiledUnroll = 8, kTiledUnrollSparse = 2;
 
 size_t tiled_build_lds(int smax) { return (size_t)smax * (sizeof(float4) + sizeof(uint32_t)); }
 
@@ -1116,17 +1119,24 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
     if (n <= 0 || ntiles <= 0) return hipSuccess;
     const size_t slot = (size_t)(uni ? ndim : ndim + 1) * sizeof(float);
     const size_t lds = (size_t)smax * (slot + (out32 ? sizeof(uint32_t) : 0));
-#define PARM_BT(DD, PT, O32, UU)                                                             \
-    hipLaunchKernelGGL((k_build_tiled<DD, kTiledLanes, kTiledUnroll, O32, UU>), dim3(ntiles), \
+    const bool sparse = 3.0 * (double)n / (double)g.ncell <= 12.0;  // atoms per stencil row
+#define PARM_BT(DD, PT, O32, UU, UNR)                                                        \
+    hipLaunchKernelGGL((k_build_tiled<DD, kTiledLanes, UNR, O32, UU>), dim3(ntiles),          \
                        dim3(kWave * kTiledLanes), lds, s, tiles, truns, smax, (const PT *)pos, \
                        (const PT *)xw, rank, diam, type, type_bits, keys, cstart, g, box,     \
                        skin, dmax, tpa, cap, npad, nbr, ncount, lcount, (PT *)xref, part,     \
                        flag, spec)
-#define PARM_BT_U(DD, PT, O32)       \
-    if (uni) {                      \
-        PARM_BT(DD, PT, O32, true);  \
-    } else {                        \
-        PARM_BT(DD, PT, O32, false); \
+#define PARM_BT_S(DD, PT, O32, UU)                       \
+    if (sparse) {                                       \
+        PARM_BT(DD, PT, O32, UU, kTiledUnrollSparse);   \
+    } else {                                            \
+        PARM_BT(DD, PT, O32, UU, kTiledUnroll);         \
+    }
+#define PARM_BT_U(DD, PT, O32)        \
+    if (uni) {                       \
+        PARM_BT_S(DD, PT, O32, true)  \
+    } else {                         \
+        PARM_BT_S(DD, PT, O32, false) \
     }
     if (ndim == 3) {
         hipLaunchKernelGGL(k_wrap<3>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
@@ -1146,6 +1156,7 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
         }
     }
 #undef PARM_BT_U
+#undef PARM_BT_S
 #undef PARM_BT
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, ntiles, part, stats,
                        flag, spec, report);
