@@ -103,16 +103,22 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
         eatom[2 * (size_t)i + 1] = nc;
     }
     if (STAGE == kStageSums) return;
+    // fused kick + drift: f and a are stored by the run's last (plain kick)
+    // launch only; nothing reads them in between
+    if constexpr (!DRIFT) {
 #pragma unroll
-    for (int k = 0; k < D; k++) A.f[k][i] = fi[k];
+        for (int k = 0; k < D; k++) A.f[k][i] = fi[k];
+    }
     if (STAGE == kStageForceOnly) return;
     const double m = A.m[i];
     const bool fixed = is_fixed(m);
     double a[D];
 #pragma unroll
     for (int k = 0; k < D; k++) a[k] = fixed ? 0.0 : fi[k] / m;
+    if constexpr (!DRIFT) {
 #pragma unroll
-    for (int k = 0; k < D; k++) A.a[k][i] = a[k];
+        for (int k = 0; k < D; k++) A.a[k][i] = a[k];
+    }
     if (STAGE == kStageForceA) return;
     double v[D];
 #pragma unroll

@@ -142,20 +142,27 @@ __device__ __forceinline__ void atom_store(const AtomArrays &A, int i, const dou
     constexpr bool DAMPED =
         (STAGE == kStageDampedKick || STAGE == kStageDampedKickDrift);
     if (STAGE == kStageSums) return;
+    // The fused kick + drift stages run inside a step loop whose last step is
+    // a plain kick launch: nothing reads f or a in between (the drift uses
+    // the a it has just computed), so they are stored by the last step only.
+    if constexpr (!DRIFT) {
 #pragma unroll
-    for (int k = 0; k < D; k++) {
-        if (STAGE == kStageForceAdd)
-            A.f[k][i] += fi[k];
-        else
-            A.f[k][i] = fi[k];
+        for (int k = 0; k < D; k++) {
+            if (STAGE == kStageForceAdd)
+                A.f[k][i] += fi[k];
+            else
+                A.f[k][i] = fi[k];
+        }
     }
     if (STAGE == kStageForceOnly || STAGE == kStageForceAdd) return;
     const bool fixed = is_fixed(in.m);
     double a[D];
 #pragma unroll
     for (int k = 0; k < D; k++) a[k] = fixed ? 0.0 : fi[k] / in.m;
+    if constexpr (!DRIFT) {
 #pragma unroll
-    for (int k = 0; k < D; k++) A.a[k][i] = a[k];
+        for (int k = 0; k < D; k++) A.a[k][i] = a[k];
+    }
     if (STAGE == kStageForceA) return;
     double v[D];
 #pragma unroll
