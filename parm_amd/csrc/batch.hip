@@ -235,7 +235,6 @@ __global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
     const int s = blockIdx.x;
     if (!force && flags[s] == 0u) return;
     __shared__ int hist[kBatchMaxCells];
-    __shared__ int ck[NT];
     __shared__ unsigned long long red[NT / kWave];
     const BatchGeom G = A.geom[s];
     const int off = G.off, n = G.n, nc0 = G.grid.nc[0], nc1 = G.grid.nc[1],
@@ -291,20 +290,38 @@ __global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
         if (t == 0) cstart[G.cell_off + ncell] = off + n;
     }
     __syncthreads();
-    // stable scatter, one chunk of NT atoms at a time
-    for (int base = 0; base < n; base += NT) {
-        const int k = base + t;
-        const int key = k < n ? keys[off + k] : -1;
-        ck[t] = key;
-        __syncthreads();
-        if (k < n) {
-            int r = 0;
-            for (int u = 0; u < t; u++) r += (ck[u] == key);
-            cellidx[off + hist[key] + r] = off + k;
+    // stable scatter, one chunk of NT atoms at a time: within a wavefront the
+    // rank among equal keys comes from ballots over the key bits; the
+    // wavefronts of the chunk then take their turn, in order, on the cell
+    // counters (the wavefront's reads of hist[key] precede, in its program
+    // order, the update by its last lane of that key)
+    {
+        constexpr int kKeyBits = 12;
+        static_assert(kBatchMaxCells <= (1 << kKeyBits), "cell keys fit the ballot match");
+        const int lane = t % kWave, wv = t / kWave;
+        const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        for (int base = 0; base < n; base += NT) {
+            const int k = base + t;
+            const bool valid = k < n;
+            const int key = valid ? keys[off + k] : 0;
+            unsigned long long eq = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < kKeyBits; b++) {
+                const bool bit = (key >> b) & 1;
+                const unsigned long long bb = __ballot(bit);
+                eq &= bit ? bb : ~bb;
+            }
+            const int r = __popcll(eq & below);
+            const bool lastk = valid && (eq >> lane) == 1ull;
+            const int cntk = __popcll(eq);
+            for (int w2 = 0; w2 < NT / kWave; w2++) {
+                if (wv == w2) {
+                    if (valid) cellidx[off + hist[key] + r] = off + k;
+                    if (lastk) hist[key] += cntk;
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads();
-        if (k < n) atomicAdd(&hist[key], 1);
-        __syncthreads();
     }
     // hist[c] is now the end of cell c (system-local sorted index).  Sorted
     // copies of the positions and diameters, and xref = x (the reference
