@@ -451,7 +451,9 @@ int build_checked(parm_system *s, const void *at) {
 int build_speculative(parm_system *s) {
     const int pstride = force_blocks(std::max(s->n, 1), 8);
     const int grid = std::min(s->tiles_cap, pstride);
-    const int limit = (int)(s->lds_max / tiled_build_lds(1));
+    // the largest stencil whose staging fits the LDS a workgroup may take
+    const size_t fixed = tiled_build_lds(0), per = tiled_build_lds(1) - fixed;
+    const int limit = s->lds_max > fixed ? (int)((s->lds_max - fixed) / per) : 0;
     const int bound = std::min(limit, s->spec_smax + std::max(s->spec_smax / 8, 64));
     // the layout the force kernel last took (accept_tiles re-decides it
     // from the plan as read; a mismatch rebuilds)

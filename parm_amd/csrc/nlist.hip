@@ -620,6 +620,8 @@ __device__ __forceinline__ void group_scan(int v, int sub, int glast, int &incl,
 // layout) instead of staged slots, for systems whose force kernel runs on the
 // untiled list.  UNI: every member has the same list diameter; the staged
 // slot then holds the offsets only (one cutoff for every candidate).
+constexpr int kBuildPad = 32;  // staged slots of padding after the stencil
+
 template <int D, bool UNI>
 struct BuildSlot {
     static constexpr int W = UNI ? D : D + 1;  // floats per slot
@@ -639,8 +641,12 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     constexpr int NT = kWave * G;
     using Slot = BuildSlot<D, UNI>;
     extern __shared__ float st_raw[];
+    // smax staged slots plus one round of padding (kBuildPad >= G * U): a
+    // round reads its U slots per lane unclamped, the ones past the range's
+    // end are masked out
     Slot *st_f = reinterpret_cast<Slot *>(st_raw);
-    uint32_t *st_e = reinterpret_cast<uint32_t *>(st_f + smax);  // OUT32 entries
+    uint32_t *st_e = reinterpret_cast<uint32_t *>(st_f + smax + kBuildPad);  // OUT32 entries
+    static_assert(G * U <= kBuildPad, "a round stays inside the padded stage");
     __shared__ TileLayout L;
     __shared__ int2 runs[kTileRuns];
     __shared__ double rimg[kTileRuns][3];
@@ -773,7 +779,7 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
 #pragma unroll
                         for (int u = 0; u < U; u++) {
                             const int s = base + u * G + sub;
-                            pw[u] = st_f[s < hi ? s : lo];
+                            pw[u] = st_f[s];
                         }
                         bool pass[U], unsure[U];
                         bool anyu = false;
@@ -1105,7 +1111,9 @@ constexpr int kTiledLanes = 4;   // lanes per atom in k_build_tiled
 // discs / spheres: ~7-9 atoms) one round of 4 x 2
 constexpr int kTiledUnroll = 8, kTiledUnrollSparse = 2;
 
-size_t tiled_build_lds(int smax) { return (size_t)smax * (sizeof(float4) + sizeof(uint32_t)); }
+size_t tiled_build_lds(int smax) {
+    return (size_t)(smax + kBuildPad) * sizeof(float4) + (size_t)smax * sizeof(uint32_t);
+}
 
 hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileRec *tiles,
                               const int2 *truns, const void *pos, void *xw, const int32_t *rank,
@@ -1118,7 +1126,8 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
                               int uni, BuildReport *report, hipStream_t s) {
     if (n <= 0 || ntiles <= 0) return hipSuccess;
     const size_t slot = (size_t)(uni ? ndim : ndim + 1) * sizeof(float);
-    const size_t lds = (size_t)smax * (slot + (out32 ? sizeof(uint32_t) : 0));
+    const size_t lds = (size_t)(smax + kBuildPad) * slot +
+                       (out32 ? (size_t)smax * sizeof(uint32_t) : 0);
     const bool sparse = 3.0 * (double)n / (double)g.ncell <= 12.0;  // atoms per stencil row
 #define PARM_BT(DD, PT, O32, UU, UNR)                                                        \
     hipLaunchKernelGGL((k_build_tiled<DD, kTiledLanes, UNR, O32, UU>), dim3(ntiles),          \
