@@ -133,11 +133,36 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     if (!RAW && S > 0) xo = PosStore<D>::load(pos[a0]);
     int wide = 0;  // some offset reaches L/4: pair differences need the fold
     int r = 0;     // run of slot s (slots rise along the loop)
-    for (int s = threadIdx.x; s < S; s += NT) {
-        while (r + 1 < kTileRuns && runs[r + 1].y <= s) r++;
-        const int j = runs[r].x + (s - runs[r].y);
-        const Vec<D> xj = PosStore<D>::load(pos[j]);
-        const int tj = SINGLE ? 0 : A.type[j];
+    // kStageBatch slots per thread at a time: their atom indices first, then
+    // all their position (and type) loads in flight together, then the stores
+    constexpr int kStageBatch = 4;
+    for (int s0 = threadIdx.x; s0 < S; s0 += kStageBatch * NT) {
+        int jb[kStageBatch];
+        PT pb[kStageBatch];
+        int tb[kStageBatch];
+#pragma unroll
+        for (int q = 0; q < kStageBatch; q++) {
+            const int s = s0 + q * NT;
+            jb[q] = 0;
+            if (s < S) {
+                while (r + 1 < kTileRuns && runs[r + 1].y <= s) r++;
+                jb[q] = runs[r].x + (s - runs[r].y);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kStageBatch; q++) {
+            if (s0 + q * NT < S) {
+                pb[q] = pos[jb[q]];
+                tb[q] = SINGLE ? 0 : A.type[jb[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kStageBatch; q++) {
+        const int s = s0 + q * NT;
+        if (s >= S) break;
+        const int j = jb[q];
+        const Vec<D> xj = PosStore<D>::load(pb[q]);
+        const int tj = tb[q];
         Vec<D> v = xj;
         if (!RAW) {
             v = offset_from<D>(xj, xo, box);
@@ -172,6 +197,7 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                 q.w = 0.0;
             }
             st64[s] = q;
+        }
         }
     }
     const bool fold = __syncthreads_or(wide) != 0;
