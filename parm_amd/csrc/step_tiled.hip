@@ -122,11 +122,9 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     const PT *pos = static_cast<const PT *>(A.pos_in);
     // the first wavefront updates the tile's atoms at the end (one lane per
     // atom); their inputs are loaded now, to arrive during the pair loop
-    // (FP64 loads them at the end instead: its pair loop needs the registers)
-    constexpr bool PREFETCH = PREC != kFP64;
     AtomIn<D> ain;
     const bool upd = STAGE != kStageSums && threadIdx.x < (unsigned)tcnt;
-    if (PREFETCH && upd) atom_load<D, STAGE>(A, a0 + (int)threadIdx.x, ain);
+    if (upd) atom_load<D, STAGE>(A, a0 + (int)threadIdx.x, ain);
     Vec<D> xo;
 #pragma unroll
     for (int k = 0; k < D; k++) xo.c[k] = 0.0;
@@ -362,7 +360,6 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
             double fa[D];
 #pragma unroll
             for (int k = 0; k < D; k++) fa[k] = fsum[k][threadIdx.x];
-            if (!PREFETCH) atom_load<D, STAGE>(A, a0 + (int)threadIdx.x, ain);
             atom_store<D, STAGE>(A, a0 + (int)threadIdx.x, fa, ain, c, top, nonfinite);
         }
     }
