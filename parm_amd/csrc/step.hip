@@ -148,6 +148,10 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
     const PT *pos = static_cast<const PT *>(A.pos_in);
     Vec<D> xi;
     if (act) xi = PosStore<D>::load(pos[i]);
+    // the integrator inputs of the atom, loaded now to arrive during the
+    // pair loop (lane sub == 0 applies the stage)
+    AtomIn<D> ain;
+    if (act && sub == 0) atom_load<D, STAGE, false>(A, i, ain);
     const bool member = act && A.rank[i] >= 0;
     if (member && Lv.built) {
         // lane sub's entries m = 0 .. nl-1 sit at (m * npad + i) * TPA + sub
@@ -194,7 +198,13 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
         for (int k = 0; k < D; k++) fi[k] += __shfl_xor(fi[k], o);
     }
 
-    atom_finish<D, STAGE, ENERGY, kBlock / kWave>(A, i, act, sub, fi, xi, c, acc, R);
+    Top2 top;
+    double nonfinite = 0.0;
+    if (act && sub == 0) {
+        ain.x = xi;
+        atom_store<D, STAGE>(A, i, fi, ain, c, top, nonfinite);
+    }
+    group_partials<D, STAGE, ENERGY, kBlock / kWave>(acc, top, nonfinite, R);
 }
 
 // Standalone first half of a step (drift + half kick) and skin test.

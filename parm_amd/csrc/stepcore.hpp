@@ -188,17 +188,6 @@ __device__ __forceinline__ void atom_store(const AtomArrays &A, int i, const dou
     for (int k = 0; k < D; k++) A.v[k][i] = v[k];
 }
 
-// Load + store for atom i (untiled kernel: the inputs are loaded at the end).
-template <int D, int STAGE>
-__device__ __forceinline__ void atom_apply(const AtomArrays &A, int i,
-                                           const double (&fi)[D], const Vec<D> &xi,
-                                           const StepCoef &c, Top2 &top, double &nonfinite) {
-    AtomIn<D> in;
-    atom_load<D, STAGE, false>(A, i, in);
-    in.x = xi;
-    atom_store<D, STAGE>(A, i, fi, in, c, top, nonfinite);
-}
-
 // This workgroup's reduction partials (all threads call it).  SKIN_W0: the
 // stage's writes (and so the skin test) were done by the first wavefront.
 template <int D, int STAGE, bool ENERGY, int NW, bool SKIN_W0 = false>
@@ -221,20 +210,6 @@ __device__ __forceinline__ void group_partials(const PairAcc &acc, Top2 top, dou
         v[kNonFinite] = nonfinite;
         block_partial<ENERGY, DRIFT, NW, SKIN_W0>(v, top, R);
     }
-}
-
-// After the TPA lanes of atom i have summed its force fi: the stage's writes
-// (lane sub == 0 of active atoms), then this workgroup's reduction partials
-// (all threads).  NW = wavefronts per workgroup.
-template <int D, int STAGE, bool ENERGY, int NW>
-__device__ __forceinline__ void atom_finish(const AtomArrays &A, int i, bool act, int sub,
-                                            const double (&fi)[D], const Vec<D> &xi,
-                                            const StepCoef &c, const PairAcc &acc,
-                                            const ReduceBuf &R) {
-    Top2 top;
-    double nonfinite = 0.0;
-    if (act && sub == 0) atom_apply<D, STAGE>(A, i, fi, xi, c, top, nonfinite);
-    group_partials<D, STAGE, ENERGY, NW>(acc, top, nonfinite, R);
 }
 
 }  // namespace parm
