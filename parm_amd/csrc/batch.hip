@@ -50,6 +50,23 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
         xi = PosStore<D>::load(pos[i]);
         box = A.geom[A.sys[i]].box;
     }
+    // the integrator inputs of the atom (lane sub == 0 applies the stage),
+    // loaded now to arrive during the pair loop
+    double m_i = 1.0, v_i[D], x0_i[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) v_i[k] = x0_i[k] = 0.0;
+    if (act && sub == 0 && STAGE != kStageSums && STAGE != kStageForceOnly) {
+        m_i = A.m[i];
+        if (STAGE != kStageForceA) {
+#pragma unroll
+            for (int k = 0; k < D; k++) v_i[k] = A.v[k][i];
+        }
+        if (DRIFT) {
+            const Vec<D> x0 = PosStore<D>::load(static_cast<const PT *>(A.xref)[i]);
+#pragma unroll
+            for (int k = 0; k < D; k++) x0_i[k] = x0.c[k];
+        }
+    }
     if (act && Lv.built) {
         const int cnt = Lv.ncount[i];
         const size_t stride = (size_t)Lv.npad * TPA;
@@ -64,9 +81,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
             row = P.tab + (size_t)A.type[i] * P.ntypes;
         int q = sub;
         size_t off = 0;
+        // the next two entries are loaded while the current two are evaluated
+        uint32_t n0 = q < cnt ? nb[0] : 0u;
+        uint32_t n1 = q + TPA < cnt ? nb[stride] : 0u;
         for (; q + TPA < cnt; q += 2 * TPA, off += 2 * stride) {
-            const uint32_t e0 = nb[off];
-            const uint32_t e1 = nb[off + stride];
+            const uint32_t e0 = n0;
+            const uint32_t e1 = n1;
+            if (q + 2 * TPA < cnt) n0 = nb[off + 2 * stride];
+            if (q + 3 * TPA < cnt) n1 = nb[off + 3 * stride];
             const PT pj0 = pos[e0 >> tb];
             const PT pj1 = pos[e1 >> tb];
             const Vec<D> r0 = diff<D>(xi, PosStore<D>::load(pj0), box);
@@ -77,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
                                                acc);
         }
         if (q < cnt) {
-            const uint32_t e0 = nb[off];
+            const uint32_t e0 = n0;
             const Vec<D> r0 = diff<D>(xi, PosStore<D>::load(pos[e0 >> tb]), box);
             visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[e0 & tmask], r0, fi, f32,
                                                acc);
@@ -110,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
         for (int k = 0; k < D; k++) A.f[k][i] = fi[k];
     }
     if (STAGE == kStageForceOnly) return;
-    const double m = A.m[i];
+    const double m = m_i;
     const bool fixed = is_fixed(m);
     double a[D];
 #pragma unroll
@@ -122,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
     if (STAGE == kStageForceA) return;
     double v[D];
 #pragma unroll
-    for (int k = 0; k < D; k++) v[k] = A.v[k][i];
+    for (int k = 0; k < D; k++) v[k] = v_i[k];
     if (!fixed) {
 #pragma unroll
         for (int k = 0; k < D; k++) v[k] = v[k] + a[k] * (DAMPED ? c.k4 : c.k2);
@@ -131,10 +153,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
         Vec<D> xn = xi;
         drift_atom<D, DAMPED>(xn, v, a, fixed, c);
         static_cast<PT *>(A.pos_out)[i] = PosStore<D>::make(xn);
-        const Vec<D> x0 = PosStore<D>::load(static_cast<const PT *>(A.xref)[i]);
         Vec<D> d;
 #pragma unroll
-        for (int k = 0; k < D; k++) d.c[k] = xn.c[k] - x0.c[k];
+        for (int k = 0; k < D; k++) d.c[k] = xn.c[k] - x0_i[k];
         A.disp[i] = sqrt(sqnorm<D>(d));
     }
 #pragma unroll
