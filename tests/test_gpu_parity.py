@@ -416,3 +416,39 @@ def test_rebuild_after_denser_packing_replans():
         s.set_forces(False)
         ref = O.forces(xs, L, t, first, last)
         assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
+
+
+@pytest.mark.parametrize("case", ["empty", "no_members"])
+def test_empty_and_memberless_systems(case):
+    """An AtomVec with no atoms, and atoms none of which is a list member:
+    the step loops (long enough to run the autotuner), rebuild counting,
+    energies and pair counts behave as the oracle's (no pairs, no forces,
+    one rebuild at initialize, free flight)."""
+    _device_or_skip()
+    print(f"edge case {case}", flush=True)
+    n = 0 if case == "empty" else 5
+    L = np.full(3, 10.0)
+    x = np.random.default_rng(0).uniform(-5, 5, size=(n, 3))
+    v = np.full((n, 3), 0.1)
+    members = None if case == "empty" else np.zeros(0, np.int32)
+    p = S.lj_params(n)
+    s = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3, members=members)
+    s.set_state(v=v, a=np.zeros_like(x))
+    t = _table(O.LJ_ATTRACT_REPULSE, p)
+    c = O.System(np.ones(n), L, t, p["diam"], 0.3, member=members)
+    c.set_state(x=x, v=v, a=np.zeros_like(x))
+    s.initialize()
+    c.initialize()
+    for kind in ("verlet", "damped"):
+        if kind == "verlet":
+            s.verlet(0.01, 20)
+            c.verlet(0.01, 20)
+        else:
+            s.damped(0.01, 0.5, 20)
+            c.damped(0.01, 0.5, 20)
+        assert s.which() == c.which()
+        assert s.numpairs() == 0
+        assert s.pair_sums()["U"] == 0.0
+        xs = s.positions()
+        assert xs.shape == (n, 3)
+        assert np.allclose(xs, c.state()[0], rtol=0, atol=1e-12)
