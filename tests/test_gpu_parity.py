@@ -452,3 +452,26 @@ def test_empty_and_memberless_systems(case):
         xs = s.positions()
         assert xs.shape == (n, 3)
         assert np.allclose(xs, c.state()[0], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("integrator", ["verlet", "damped"])
+def test_nonfinite_position_is_reported(integrator):
+    """A velocity set to +inf from the host drives one position to +inf in
+    the first half step; the step loop reports PARM_E_NONFINITE (every
+    kernel index comes from the list or is clamped, so the kernels run to
+    completion on the non-finite values)."""
+    _device_or_skip()
+    n = 2000
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=21)
+    p = S.lj_params(n)
+    s = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3)
+    v = np.zeros_like(x)
+    v[7, 1] = np.inf
+    s.set_state(v=v, a=np.zeros_like(x))
+    s.initialize()
+    with pytest.raises(capi.ParmError) as e:
+        if integrator == "verlet":
+            s.verlet(0.002, 30)
+        else:
+            s.damped(0.002, 0.5, 30)
+    assert e.value.code == capi.E_NONFINITE, str(e.value)
