@@ -418,19 +418,20 @@ def test_rebuild_after_denser_packing_replans():
         assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
 
 
-@pytest.mark.parametrize("case", ["empty", "no_members"])
+@pytest.mark.parametrize("case", ["empty", "no_members", "single"])
 def test_empty_and_memberless_systems(case):
-    """An AtomVec with no atoms, and atoms none of which is a list member:
-    the step loops (long enough to run the autotuner), rebuild counting,
-    energies and pair counts behave as the oracle's (no pairs, no forces,
-    one rebuild at initialize, free flight)."""
+    """An AtomVec with no atoms, atoms none of which is a list member, and a
+    single member atom: the step loops (long enough to run the autotuner),
+    rebuild counting, energies and pair counts behave as the oracle's (no
+    pairs, no forces, free flight; the lone atom's top-2 displacement is
+    (d, 0), so it rebuilds once d reaches the skin)."""
     _device_or_skip()
     print(f"edge case {case}", flush=True)
-    n = 0 if case == "empty" else 5
+    n = {"empty": 0, "no_members": 5, "single": 1}[case]
     L = np.full(3, 10.0)
     x = np.random.default_rng(0).uniform(-5, 5, size=(n, 3))
     v = np.full((n, 3), 0.1)
-    members = None if case == "empty" else np.zeros(0, np.int32)
+    members = np.zeros(0, np.int32) if case == "no_members" else None
     p = S.lj_params(n)
     s = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3, members=members)
     s.set_state(v=v, a=np.zeros_like(x))
