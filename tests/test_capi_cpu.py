@@ -54,3 +54,16 @@ def test_batch_no_gpu_means_loud_failure():
         capi.Batch(3, [4, 8], [[2.0] * 3, [3.0] * 3], 0.2)
     with pytest.raises(ValueError):
         capi.Batch(3, [4, 0], [[2.0] * 3, [3.0] * 3], 0.2)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message before
+    touching any device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "exactly one GPU" in r.stderr
+    assert r.stdout == ""
