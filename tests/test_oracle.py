@@ -426,6 +426,76 @@ def test_damped_free_particle_closed_form():
     assert np.allclose(x, x0 + v0 * (1 - e) / g, rtol=1e-11, atol=1e-13)
 
 
+def test_damped_without_damping_is_verlet_bitwise():
+    """CollectionDamped with damping <= 0 has c0 = c1 = 1, c2 = 1/2
+    (src/collection.cpp:342-348), and then every update of its step
+    (:356-381) rounds exactly as CollectionVerlet's (:442-469): (0.5 dt) dt
+    = (dt dt)/2 and v*1 = v exactly.  The two trajectories of an LJ liquid
+    (with fixed atoms and rebuilds) must agree bit for bit; this pins the
+    acceleration terms a c2 dt^2, a dt (c1 - c2) and a dt c2."""
+    n = 64
+    x, L, types = _random_liquid(n, 0.8, 3, 11)
+    masses = np.ones(n)
+    masses[3] = np.inf
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=(n, 3))
+    t = O.PairTable(O.LJ_ATTRACT_REPULSE, [[1.0]], [1.0], [2.5], types)
+    runs = []
+    for kind in ("verlet", "damped"):
+        s = O.System(masses, L, t, np.full(n, 2.5), 0.3)
+        s.set_state(x=x, v=v, a=np.zeros_like(x))
+        s.initialize()
+        for _ in range(60):
+            if kind == "verlet":
+                s.verlet(0.003, 1)
+            else:
+                s.damped(0.003, 0.0, 1)
+        runs.append((s.state(), s.which()))
+    (sa, wa), (sb, wb) = runs
+    assert wa == wb and wa > 1
+    for qa, qb in zip(sa, sb):
+        assert np.array_equal(qa, qb)
+
+
+def _damped_pair_run(dt, T, gamma, sig, u0):
+    """Two unit-mass atoms in harmonic contact (RepulsionPair exponent 2,
+    eps = 1) on the x axis, released from rest at separation sig + u0 < sig;
+    CollectionDamped with damping gamma up to time T.  Returns the
+    separation at T."""
+    t = O.PairTable(O.REPULSION, [1.0], [sig], [2.0], [0, 0])
+    s = O.System([1.0, 1.0], [50.0] * 3, t, [sig, sig], 0.2)
+    R0 = sig + u0
+    s.set_state(x=np.array([[R0 / 2, 0, 0], [-R0 / 2, 0, 0]]), v=np.zeros((2, 3)),
+                a=np.zeros((2, 3)))
+    s.initialize()
+    s.damped(dt, gamma, int(round(T / dt)))
+    x, _, _, _ = s.state()
+    return x[0, 0] - x[1, 0]
+
+
+def test_damped_harmonic_contact_converges_to_analytic_oscillator():
+    """The separation R of a harmonic contact obeys u'' + gamma u' + w^2 u = 0
+    with u = R - sig and w^2 = 2 eps / (sig^2 m) (force eps (1 - R/sig)/sig on
+    each atom, relative acceleration twice that; each atom's velocity damped
+    at rate gamma).  Overdamped from rest the contact never opens, so the
+    closed form holds for all t.  The damped Verlet step is second order: the
+    error at fixed T drops ~4x per halving of dt.  A c2 <-> c1 slip makes it
+    first order (or inconsistent) and fails the ratio."""
+    sig, gamma, u0, T = 1.0, 3.0, -0.2, 2.0
+    w2 = 2.0 / (sig * sig)
+    disc = math.sqrt(gamma * gamma - 4 * w2)
+    r1, r2 = (-gamma + disc) / 2, (-gamma - disc) / 2
+    # u(0) = u0, u'(0) = 0
+    A = u0 * r2 / (r2 - r1)
+    B = -u0 * r1 / (r2 - r1)
+    exact = sig + A * math.exp(r1 * T) + B * math.exp(r2 * T)
+    errs = [abs(_damped_pair_run(dt, T, gamma, sig, u0) - exact)
+            for dt in (0.02, 0.01, 0.005)]
+    assert errs[0] < 1e-3
+    for e_big, e_small in zip(errs, errs[1:]):
+        assert 3.3 < e_big / e_small < 4.7, errs
+
+
 def test_observables_closed_form():
     t = O.PairTable(O.REPULSION, [1.0], [1.0], [2.0], np.zeros(4, np.int32))
     masses = [1.0, 2.0, np.inf, 0.5]
