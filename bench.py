@@ -9,10 +9,13 @@ Gaussian velocities at T = 1 with zero momentum, neighbour skin 0.3,
 velocity Verlet dt = 0.002, NVE.  A "step" is one CollectionVerlet::timestep()
 including the neighbour-list skin test and any rebuild.
 
-The timed region runs `--steps` steps in FP64 (the value) after `--warmup`
-untimed steps, bracketed by device synchronisation.  The same system is then
-run in mixed precision (FP32 pair arithmetic, FP64 state) over the same steps
-and its throughput, energy drift and force error are reported beside it.
+The step kernel is autotuned first (timed on its own, outside the timed
+region); then `--warmup` untimed steps, then exactly `--steps` timed FP64
+steps (the value) bracketed by device synchronisation.  The same system is
+then run in mixed precision (FP32 pair arithmetic, FP64 state) over the same
+steps.  Untimed beside them: 10^4-step NVE runs in both precisions (after a
+1000-step equilibration from the pushed-apart start) for the energy drift and
+the steady-state throughput, and the per-atom force error of the mixed form.
 
 Roofline: the fused step kernel (forces + kick + next drift + skin test) is
 timed with HIP events on its own stream; its algorithmic bytes per launch are
@@ -20,12 +23,14 @@ N * (192 + 4 * nbar) with nbar the mean full-list length (FP64 state read and
 written once, one int32 list index per entry, neighbour gathers counted as
 cache hits).  Peak: 8 TB/s HBM3E.
 
-Batched packings (reported under "batched"): 4096 cubic systems with atom
-counts uniform in [64, 4096], bidisperse 1.0 / 1.4 harmonic spheres at
-packing fraction 0.64 from uniform random positions, damped Verlet dt = 0.01
-(damping 1.0), 2000 steps, mixed precision, all systems in one batch with
-graph-replayed steps; beside it the same systems run one after another
-through the single-system path (a sample of them).
+Also in the same line: the 1000-atom LJ case the reference runs on its CPU
+("lj1000", beside the oracle's full run of it), the 262,144-disk 2D packing
+relaxation ("relax2d") and the batched packings ("batched"): 4096 cubic
+systems with atom counts uniform in [64, 4096], bidisperse 1.0 / 1.4 harmonic
+spheres at packing fraction 0.64 from uniform random positions, damped Verlet
+dt = 0.01 (damping 1.0), 2000 steps, mixed precision, all systems in one
+batch with graph-replayed steps; beside it the same systems run one after
+another through the single-system path (a sample of them).
 
 CPU baseline: the oracle's restatement of ParM's single-core CPU path
 (AoS atoms, half list, O(N^2) rebuild), timed on this host on a bounded
@@ -79,36 +84,75 @@ def total_energy(s):
     return s.pair_sums()["U"] + s.kinetic_energy()
 
 
-def run_timed(s, dt, steps, warmup, drift_every=0):
-    """Returns wall seconds, profile dict and the energy trace."""
+def run_timed(s, dt, steps, warmup):
+    """Autotunes the step kernel (timed on its own), runs `warmup` untimed
+    steps, then times exactly `steps` steps bracketed by device
+    synchronisation.  Returns wall seconds, the tuning seconds and the profile
+    dict of the timed steps."""
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.autotune(dt)
+    t_tune = time.perf_counter() - t0
     if warmup:
         s.verlet(dt, warmup)
     s.synchronize()
-    energies = []
     s.profile_read(reset=True)
     s.profile_enable(True)
-    if drift_every:
-        energies.append(total_energy(s))
     t0 = time.perf_counter()
-    done = 0
-    wall_e = 0.0
-    while done < steps:
-        k = min(steps - done, drift_every) if drift_every else steps
-        s.verlet(dt, k)
-        done += k
-        if drift_every:
-            te = time.perf_counter()
-            energies.append(total_energy(s))
-            wall_e += time.perf_counter() - te
+    s.verlet(dt, steps)
     s.synchronize()
-    wall = time.perf_counter() - t0 - wall_e
+    wall = time.perf_counter() - t0
     prof = s.profile_read(reset=True)
     s.profile_enable(False)
-    return wall, prof, energies
+    return wall, t_tune, prof
+
+
+def drift_run(x, v, L, prec, skin, dt, equil, steps, every):
+    """NVE energy conservation: `equil` untimed steps from the initial state,
+    then `steps` steps with the total energy sampled every `every` steps
+    (sampling outside the timing).  Returns the drift after equilibration
+    (max |E(t) - E(t0)| / |E(t0)|), the drift from step 0 and the run's own
+    throughput."""
+    s = make_lj(x, v, L, prec, skin)
+    s.autotune(dt)
+    E = [total_energy(s)]
+    done = 0
+    while done < equil:
+        k = min(every, equil - done)
+        s.verlet(dt, k)
+        done += k
+        E.append(total_energy(s))
+    i0 = len(E) - 1
+    w0 = s.which()
+    wall = 0.0
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.verlet(dt, k)
+        s.synchronize()
+        wall += time.perf_counter() - t0
+        done += k
+        E.append(total_energy(s))
+    rebuilds = s.which() - w0
+    s.close()
+    E = np.asarray(E)
+    e0 = E[i0]
+    return {
+        "equilibration_steps": equil, "steps": steps, "sample_every": every,
+        "energy_drift_rel_max": float(np.max(np.abs(E[i0:] - e0)) / abs(e0)),
+        "energy_drift_rel_max_from_step0": float(np.max(np.abs(E - E[0])) / abs(E[0])),
+        "E_start": float(e0), "E_end": float(E[-1]),
+        "value": len(x) * steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "steps_per_rebuild": steps / max(rebuilds, 1),
+    }
 
 
 def cpu_baseline(x, v, L, skin, dt, gpu_steps_per_rebuild):
-    """Time the oracle (single core) on a bounded sample of the same workload."""
+    """Time the oracle (single core) on a bounded sample of the same workload.
+    The rebuild cost is amortised over the steady-state steps per rebuild of
+    the GPU's 10^4-step run (the same trajectory; the skin rule decides)."""
     from oracle import oracle as O
 
     n = len(x)
@@ -143,8 +187,8 @@ def cpu_baseline(x, v, L, skin, dt, gpu_steps_per_rebuild):
         "sample": (f"oracle restatement of ParM's CPU path, 1 core: {nsteps} non-rebuild "
                    f"Verlet steps at N={n} measured ({t_step:.3f} s/step); O(N^2) rebuild "
                    f"measured at N={nr} ({t_reb_small:.3f} s) and extrapolated by N^2 to "
-                   f"{t_reb:.0f} s, amortised over the GPU run's {gpu_steps_per_rebuild:.1f} "
-                   f"steps per rebuild"),
+                   f"{t_reb:.0f} s, amortised over {gpu_steps_per_rebuild:.2f} steps per "
+                   f"rebuild (the GPU's 10^4-step FP64 run)"),
     }
 
 
@@ -179,8 +223,11 @@ def batched(args):
     b.set_state(x=x, v=np.zeros_like(x), a=np.zeros_like(x))
     b.initialize()
     U0, _ = b.sums()
-    # autotune (first call of >= 16 steps) and graph capture happen here;
-    # 17 steps = 8 graph replays, leaving the position buffers as captured
+    # autotune, then graph capture (17 steps = 8 graph replays, leaving the
+    # position buffers as captured)
+    t0 = time.perf_counter()
+    b.autotune()
+    t_tune = time.perf_counter() - t0
     b.damped(args.batch_dt, args.batch_damping, 17)
     b.set_state(x=x, v=np.zeros_like(x), a=np.zeros_like(x))
     b.initialize()
@@ -189,8 +236,22 @@ def batched(args):
     b.damped(args.batch_dt, args.batch_damping, args.batch_steps)
     U1, _ = b.sums()  # synchronises
     wall = time.perf_counter() - t0
-    w1, _ = b.lists()
+    w1, npairs = b.lists()
+    sums_end = b.pair_sums()
+    # roofline of the dominant kernel: the same fused force kernel launched
+    # eagerly with HIP events around each launch, over a further stretch of
+    # the same run (the graph replays launch the same kernels)
+    b.profile_read(reset=True)
+    b.profile_enable(True)
+    b.damped(args.batch_dt, args.batch_damping, args.batch_profile_steps, graph=False)
+    prof = b.profile_read(reset=True)
+    b.profile_enable(False)
+    _, npairs_p = b.lists()
     b.close()
+    nbar = 2.0 * float(np.sum(npairs_p)) / ntot
+    B = 192.0 + 4.0 * nbar
+    k_us = 1e3 * prof["force_ms"] / max(prof["force_launches"], 1)
+    achieved = ntot * B / (k_us * 1e-6) / 1e9
     value = ntot * args.batch_steps / wall
     log(f"bench: batched {value:.3e} atom-steps/s, {len(sizes) / wall:.1f} packings/s, "
         f"{1e3 * wall / args.batch_steps:.3f} ms/step")
@@ -232,6 +293,17 @@ def batched(args):
                        "sample": f"the first {k} systems, one parm_system each, "
                                  f"{args.batch_steps} steps each, run one after another"},
         "speedup_vs_sequential": value / seq_value,
+        "autotune_s": t_tune,
+        "virial_per_atom_end": float(sums_end["W"].sum() / ntot),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_batch_force (fused force + kick + drift, all systems)",
+                     "kernel_avg_us": k_us, "neighbours_per_atom": nbar,
+                     "bytes_per_atom_step": B,
+                     "rebuild_avg_us": 1e3 * prof["rebuild_ms"] /
+                     max(prof["rebuild_launches"], 1),
+                     "sample": f"{prof['force_launches']} eager launches after the timed "
+                               f"run, HIP events on the batch's stream"},
     }
 
 
@@ -431,19 +503,50 @@ def lj1000(args):
     return out
 
 
+def host_cpu():
+    """Model name of the host CPU and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def pinned_to_one_core(fn, *a, **kw):
+    """Runs fn pinned to one core (the reference is single-core), restoring
+    the affinity afterwards.  Returns (result, core)."""
+    before = os.sched_getaffinity(0)
+    core = min(before)
+    os.sched_setaffinity(0, {core})
+    try:
+        return fn(*a, **kw), core
+    finally:
+        os.sched_setaffinity(0, before)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--natoms", type=int, default=1 << 20)
     ap.add_argument("--skin", type=float, default=0.3)
     ap.add_argument("--dt", type=float, default=0.002)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-drift", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=10000)
+    ap.add_argument("--drift-equil", type=int, default=1000)
     ap.add_argument("--drift-every", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 1000-atom LJ and 2D relaxation workloads")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--batch-dt", type=float, default=0.01)
@@ -451,9 +554,10 @@ def main():
     ap.add_argument("--batch-skin", type=float, default=0.2)
     ap.add_argument("--batch-precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--batch-seq-sample", type=int, default=32)
+    ap.add_argument("--batch-profile-steps", type=int, default=40)
     ap.add_argument("--workload", choices=["lj", "relax2d", "lj1000"], default="lj",
-                    help="relax2d: the 2D packing relaxation; lj1000: the 1000-atom LJ "
-                         "case beside the oracle's full CPU run (each its own JSON line)")
+                    help="lj (default): every workload in one JSON line; relax2d / lj1000: "
+                         "only that workload, as its own JSON line")
     ap.add_argument("--relax-natoms", type=int, default=262144)
     ap.add_argument("--relax-precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--relax-dt", type=float, default=0.3)
@@ -485,23 +589,18 @@ def main():
     x, v, L = workloads.lj_liquid(n, rho=0.8, T=1.0, min_sep=0.8, seed=args.seed, log=log)
     log(f"bench: workload ready in {time.perf_counter() - t0:.1f} s, L={L[0]:.4f}")
 
-    # ---------------- FP64 (the value)
+    # ---------------- FP64 (the value): exactly --steps timed steps
     s = make_lj(x, v, L, capi.FP64, args.skin)
     E0 = total_energy(s)
-    wall, prof, energies = run_timed(s, args.dt, args.steps, args.warmup,
-                                     drift_every=args.drift_every)
+    wall, t_tune, prof = run_timed(s, args.dt, args.steps, args.warmup)
     np_pairs = s.numpairs()
     nbar = 2.0 * np_pairs / n
     layout = s.list_info()
     ms_per_step = 1e3 * wall / args.steps
     value = n * args.steps / wall
-    e = np.asarray(energies)
-    drift64 = float(np.max(np.abs(e - e[0])) / abs(e[0])) if len(e) else None
     k_avg_ms = prof["step_ms"] / max(prof["step_launches"], 1)
     B = 192.0 + 4.0 * nbar
     achieved = n * B / (k_avg_ms * 1e-3) / 1e9
-    steps_per_rebuild = (args.steps / prof["rebuilds"]) if prof["rebuilds"] else float(
-        args.steps)
     key = (layout["tiled"], layout["lanes"])
     traffic = (TRAFFIC_BYTES_PER_ATOM[key] * n
                if n == 1 << 20 and args.skin == 0.3 and key in TRAFFIC_BYTES_PER_ATOM
@@ -510,8 +609,7 @@ def main():
         f"{k_avg_ms * 1e3:.1f} us ({'tiled' if layout['tiled'] else 'untiled'} list, "
         f"{layout['lanes']} lanes/atom), {prof['rebuilds']} rebuilds "
         f"({prof['rebuild_ms'] / max(prof['rebuilds'], 1):.3f} ms each), nbar {nbar:.1f}, "
-        f"drift {drift64}")
-    xs_end = None
+        f"autotune {t_tune:.3f} s")
     s.close()
 
     out = {
@@ -532,8 +630,8 @@ def main():
                                "plain FP64", "natoms": n, "density": 0.8,
                    "cutoff": 2.5, "skin": args.skin, "dt": args.dt, "T0": 1.0,
                    "E0": E0, "neighbours_per_atom": nbar,
-                   "steps_per_rebuild": steps_per_rebuild,
-                   "energy_drift_rel_max": drift64, "parallelism": "single GPU"},
+                   "rebuilds_in_timed_steps": prof["rebuilds"],
+                   "autotune_s": t_tune, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("k_force_tiled" if layout["tiled"] else "k_force") +
@@ -544,13 +642,10 @@ def main():
                      "bytes_per_atom_step": B},
     }
 
-    # ---------------- mixed precision (same system, same seed)
+    # ---------------- mixed precision (same system, same seed, same steps)
     if not args.no_mixed:
         sm = make_lj(x, v, L, capi.MIXED, args.skin)
-        wall_m, prof_m, en_m = run_timed(sm, args.dt, args.steps, args.warmup,
-                                         drift_every=args.drift_every)
-        em = np.asarray(en_m)
-        drift_m = float(np.max(np.abs(em - em[0])) / abs(em[0]))
+        wall_m, t_tune_m, prof_m = run_timed(sm, args.dt, args.steps, args.warmup)
         value_m = n * args.steps / wall_m
         lay_m = sm.list_info()
         sm.close()
@@ -567,20 +662,47 @@ def main():
             "list_layout": "tiled" if lay_m["tiled"] else "untiled",
             "lanes_per_atom": lay_m["lanes"],
             "speedup_vs_fp64": value_m / value,
-            "energy_drift_rel_max": drift_m,
-            "force_err_abs_max": float(d.max()), "force_err_abs_rms": float(np.sqrt(np.mean(d * d))),
+            "autotune_s": t_tune_m,
+            "force_err_abs_max": float(d.max()),
+            "force_err_abs_rms": float(np.sqrt(np.mean(d * d))),
             "force_err_rel_rms": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(fn * fn))),
             "kernel_avg_us": 1e3 * prof_m["step_ms"] / max(prof_m["step_launches"], 1),
         }
-        log(f"bench: mixed {value_m:.3e} atom-steps/s ({value_m / value:.2f}x), drift "
-            f"{drift_m:.2e}")
+        log(f"bench: mixed {value_m:.3e} atom-steps/s ({value_m / value:.2f}x)")
+
+    # ---------------- NVE drift over 10^4 steps, both precisions (untimed legs)
+    steady_spr = None
+    if not args.no_drift:
+        drift = {}
+        for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+            if name == "mixed" and args.no_mixed:
+                continue
+            log(f"bench: {name} drift run ({args.drift_equil} + {args.drift_steps} steps)")
+            drift[name] = drift_run(x, v, L, prec, args.skin, args.dt, args.drift_equil,
+                                    args.drift_steps, args.drift_every)
+            log(f"bench: {name} drift {drift[name]['energy_drift_rel_max']:.2e}, "
+                f"{drift[name]['value']:.3e} atom-steps/s over the long run")
+        if "fp64" in drift and "mixed" in drift:
+            drift["mixed_speedup_vs_fp64"] = drift["mixed"]["value"] / drift["fp64"]["value"]
+        out["nve_drift"] = drift
+        steady_spr = drift["fp64"]["steps_per_rebuild"]
+
+    if not args.no_extra:
+        log("bench: 1000-atom LJ (the reference's CPU-runnable case)")
+        out["lj1000"] = lj1000(args)
+        log("bench: 2D packing relaxation")
+        out["relax2d"] = relax2d(args)
 
     if not args.no_batch:
         out["batched"] = batched(args)
 
     if not args.no_cpu:
-        log("bench: timing the CPU baseline sample")
-        out["cpu_baseline"] = cpu_baseline(x, v, L, args.skin, args.dt, steps_per_rebuild)
+        log("bench: timing the CPU baseline sample (one core)")
+        spr = steady_spr if steady_spr else 7.0
+        cb, core = pinned_to_one_core(cpu_baseline, x, v, L, args.skin, args.dt, spr)
+        model, ncores = host_cpu()
+        cb["host"] = f"{model}; pinned to core {core} of {ncores} available"
+        out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -4,11 +4,16 @@
  *
  * One `parm_system` is one ParM AtomVec (src/box.hpp:441-479) with its
  * OriginBox (src/box.hpp:97-157), one NeighborList (src/trackers.hpp:157-214)
- * and one neighbour-listed pair interaction, either
+ * and one neighbour-listed pair interaction (src/interaction.hpp:1876-1945),
+ * one of the pair kinds of enum parm_pair_kind:
  *   NListed<IEpsSigCutAtom, LJAttractRepulsePair>  (SWIG name LJAttractRepulse)
  *   NListed<EpsSigExpAtom,  RepulsionPair>         (SWIG name Repulsion)
- * (src/interaction.hpp:1876-1945, 989-1018, 1251-1299, 1454-1466, 1528-1590;
- * src/sim.i:629,635), driven by CollectionVerlet / CollectionDamped
+ *   NListed<EpsSigCutAtom,  LennardJonesCutPair>   (src/bin/LJatoms.cpp)
+ *   NListed<IEpsISigCutAtom, LennardJonesCutPair>  (SWIG name LJIICut)
+ *   NListed<EpsSigAtom,     LJRepulsePair>         (SWIG name LJRepulse)
+ *   NListed<IEpsISigExpAtom, RepulsionPair>        (SWIG name RepulsionII)
+ * (src/interaction.hpp:857-1018, 1251-1299, 1454-1590; src/sim.i:621-636),
+ * driven by CollectionVerlet / CollectionDamped
  * (src/collection.hpp:273-295, 360-374).  All state lives on the current HIP
  * device at create time; every call is ordered on the system's own stream.
  *
@@ -50,8 +55,12 @@ enum parm_precision { PARM_FP64 = 0, PARM_MIXED = 1 };
 
 enum parm_pair_kind {
     PARM_PAIR_NONE = 0,
-    PARM_PAIR_LJ_ATTRACT_REPULSE = 1, /* NListed<IEpsSigCutAtom, LJAttractRepulsePair> */
-    PARM_PAIR_REPULSION = 2           /* NListed<EpsSigExpAtom, RepulsionPair>        */
+    PARM_PAIR_LJ_ATTRACT_REPULSE = 1, /* NListed<IEpsSigCutAtom, LJAttractRepulsePair>  */
+    PARM_PAIR_REPULSION = 2,          /* NListed<EpsSigExpAtom, RepulsionPair>         */
+    PARM_PAIR_LJ_CUT = 3,             /* NListed<EpsSigCutAtom, LennardJonesCutPair>   */
+    PARM_PAIR_LJ_CUT_INDEXED = 4,     /* NListed<IEpsISigCutAtom, LennardJonesCutPair> */
+    PARM_PAIR_LJ_REPULSE = 5,         /* NListed<EpsSigAtom, LJRepulsePair>            */
+    PARM_PAIR_REPULSION_INDEXED = 6   /* NListed<IEpsISigExpAtom, RepulsionPair>       */
 };
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -76,6 +85,9 @@ int parm_system_info(const parm_system *sys, int *ndim, int *precision,
 
 /* OriginBox(Vec size) (src/box.hpp:102): edge lengths L[ndim] > 0. */
 int parm_set_box(parm_system *sys, const double *L);
+/* OriginBox::box_shape() (src/box.hpp:152): the current edges (the
+ * minimiser below resizes the box). */
+int parm_get_box(parm_system *sys, double *L);
 
 /* Atom.x / .v / .a / .f assignment (src/box.hpp:234-249): any pointer may be
  * NULL to leave that field.  Uploads are ordered before later work. */
@@ -104,6 +116,16 @@ int parm_nlist_set(parm_system *sys, double skin, int64_t nmembers,
  * list was rebuilt. */
 int parm_nlist_update(parm_system *sys, int force, int *rebuilt);
 
+/* NeighborList::ignore(a[k], b[k]) for k = 0..npairs-1 (src/trackers.hpp:
+ * 190-193; PairList::add_pair :76-83): the unordered pairs of atom indices
+ * never enter the list (the predicate skips them, src/trackers.cpp:64), and
+ * the next update rebuilds (ignorechanged).  Pairs already ignored are kept
+ * once.  parm_nlist_set starts a new list with no exclusions. */
+int parm_nlist_ignore(parm_system *sys, int64_t npairs, const int32_t *a,
+                      const int32_t *b);
+/* NeighborList::ignore_size() (src/trackers.hpp:204): distinct ignored pairs. */
+int parm_nlist_ignore_size(parm_system *sys, int64_t *n);
+
 /* NeighborList::which() (rebuild counter) and numpairs() (trackers.hpp:187-188). */
 int parm_nlist_which(parm_system *sys, uint32_t *which);
 int parm_nlist_numpairs(parm_system *sys, int64_t *npairs);
@@ -122,7 +144,9 @@ int parm_nlist_pairs(parm_system *sys, int64_t cap, int32_t *first,
  * epsilons row eps_rows[t*ntypes .. +ntypes), indx t, sigma[t], sigcut[t]
  * (IEpsSigCutAtom(a, epsilons, indx, sigma, cut), interaction.hpp:997-1003).
  * Pair parameters follow LJAttractRepulsePair's constructor
- * (interaction.hpp:1256-1269).  ntypes <= 64. */
+ * (interaction.hpp:1256-1269).  ntypes <= 64.  The rows must be symmetric
+ * (IEpsSigCutAtom::get_epsilon asserts it, interaction.hpp:1010-1016), else
+ * PARM_E_UNSUPPORTED. */
 int parm_pair_lj_attract_repulse(parm_system *sys, int ntypes,
                                  const double *eps_rows, const double *sigma,
                                  const double *sigcut,
@@ -134,6 +158,44 @@ int parm_pair_lj_attract_repulse(parm_system *sys, int ntypes,
 int parm_pair_repulsion(parm_system *sys, int ntypes, const double *eps,
                         const double *sigma, const double *exponent,
                         const int32_t *atom_type);
+
+/* NListed<EpsSigCutAtom, LennardJonesCutPair> (the pair src/bin/LJatoms.cpp:
+ * 37-38 uses): type t carries eps[t], sigma[t], sigcut[t]
+ * (EpsSigCutAtom(a, epsilon, sigma, cut), interaction.hpp:897-905); pair
+ * parameters eps = sqrt(e1 e2), sigma = (s1 + s2) / 2, cut = max(sigcut) and
+ * the LennardJonesCut energy shift (interaction.hpp:238-281, 967-987).
+ * List diameter of an atom: sigma * sigcut.  ntypes <= 64. */
+int parm_pair_lj_cut(parm_system *sys, int ntypes, const double *eps,
+                     const double *sigma, const double *sigcut,
+                     const int32_t *atom_type);
+
+/* NListed<IEpsISigCutAtom, LennardJonesCutPair> (SWIG name LJIICut,
+ * src/sim.i:625): epsilons and sigmas as ntypes x ntypes rows, pair (t1, t2)
+ * takes eps_rows[t1*ntypes+t2] and sigma_rows[t1*ntypes+t2]
+ * (IEpsISigCutAtom::get_epsilon / get_sigma, interaction.hpp:911-956);
+ * cut = max(sigcut).  List diameter: max of the atom's sigma row * sigcut.
+ * Rows must be symmetric (each pair is evaluated from both atoms), else
+ * PARM_E_UNSUPPORTED. */
+int parm_pair_lj_cut_indexed(parm_system *sys, int ntypes, const double *eps_rows,
+                             const double *sigma_rows, const double *sigcut,
+                             const int32_t *atom_type);
+
+/* NListed<EpsSigAtom, LJRepulsePair> (SWIG name LJRepulse, src/sim.i:621):
+ * type t carries eps[t], sigma[t] (EpsSigAtom(a, epsilon, sigma),
+ * interaction.hpp:857-866); eps = sqrt(e1 e2), sigma = (s1 + s2) / 2, the
+ * purely repulsive LJ cut at sigma (LJRepulsive, interaction.hpp:119-147,
+ * 875-891).  List diameter: sigma. */
+int parm_pair_lj_repulse(parm_system *sys, int ntypes, const double *eps,
+                         const double *sigma, const int32_t *atom_type);
+
+/* NListed<IEpsISigExpAtom, RepulsionPair> (SWIG name RepulsionII,
+ * src/sim.i:636): ntypes x ntypes eps and sigma rows as in
+ * parm_pair_lj_cut_indexed, exponent[t] per type, n = (n1 + n2) / 2
+ * (interaction.hpp:1477-1521, 1537-1542).  List diameter: max of the atom's
+ * sigma row.  Rows must be symmetric. */
+int parm_pair_repulsion_indexed(parm_system *sys, int ntypes, const double *eps_rows,
+                                const double *sigma_rows, const double *exponent,
+                                const int32_t *atom_type);
 
 /* ------------------------------------------------------ forces, sums */
 
@@ -182,6 +244,68 @@ int parm_damped_steps(parm_system *sys, double dt, double damping,
 
 /* Wait for all work queued on the system. */
 int parm_synchronize(parm_system *sys);
+
+/* No reference counterpart (performance tuning).  Picks the step kernel's
+ * neighbour-list layout and lanes per atom now, by timing the candidates on
+ * the current list (outputs redirected to scratch: the state is unchanged).
+ * Without this call the first parm_verlet_steps / parm_damped_steps call of
+ * >= 16 steps does it.  damped selects the kernel of that integrator. */
+int parm_autotune(parm_system *sys, double dt, int damped);
+
+/* ------------------------------------------------ nonlinear CG minimiser */
+/* CollectionNLCG(box, atoms, dt, P0, {pair}, {list}, {}, kappa, kmax, secmax,
+ * seceps) (src/collection.hpp:400-474, src/collection.cpp:494-523) on the
+ * system: the pressure-controlled conjugate-gradient packing minimiser of
+ * src/bin/packer.cpp:102 and pyparm/packmin.py:69, with the box resized in
+ * place.  Creation runs Collection::initialize (src/collection.cpp:13-19).
+ * fixed_L != 0 gives CollectionNLCGFixedL (only the first box edge changes,
+ * src/collection.cpp:855-870).  The system must outlive the minimiser. */
+typedef struct parm_nlcg parm_nlcg;
+int parm_nlcg_create(parm_system *sys, double dt, double P0, double kappa, double kmax,
+                     int secmax, double seceps, int fixed_L, parm_nlcg **out);
+int parm_nlcg_destroy(parm_nlcg *c);
+/* set_max_alpha, set_max_alpha_fraction, set_max_dx, set_max_step and the
+ * maxdV member (src/collection.hpp:413-426, 473-477; defaults 2, 0, 100,
+ * 1e-3, 0). */
+int parm_nlcg_set_limits(parm_nlcg *c, double alphamax, double afrac, double dxmax,
+                         double stepmax, double maxdV);
+/* set_dt + set_pressure_goal + set_kappa (each ends in reset(),
+ * src/collection.hpp:458-472). */
+int parm_nlcg_set_params(parm_nlcg *c, double dt, double P0, double kappa);
+/* reset(), set_forces(constraints_and_a, setV), timestep() x nsteps and
+ * descend() (src/collection.cpp:528-569, 685-853). */
+int parm_nlcg_reset(parm_nlcg *c);
+int parm_nlcg_set_forces(parm_nlcg *c, int constraints_and_a, int setV);
+int parm_nlcg_timestep(parm_nlcg *c, int64_t nsteps);
+int parm_nlcg_descend(parm_nlcg *c);
+/* hamiltonian() = U + P0 V, pressure() = sum r.f / (NDIM V) and
+ * kinetic_energy() (masses ignored) (src/collection.cpp:571-606); NULL skips. */
+int parm_nlcg_observables(parm_nlcg *c, double *H, double *P, double *K);
+/* The public tracking members, in this order: alpha, beta, betaused, dxsum,
+ * alphavmax, sec, k, vl, fl, al, Knew, maxdV (12 doubles). */
+int parm_nlcg_info(parm_nlcg *c, double *out);
+
+/* ---------------------------------------------------- snapshots for output */
+/* No reference counterpart as an interface: the state output of the
+ * reference's drivers and Python pickling (src/bin/LJatoms.cpp:130-158,
+ * src/sim.i:494-575) read Atom.x / .v on the host; here that copy leaves the
+ * device without stalling the step loop.  begin queues, on the system's
+ * stream, a gather of the positions (what bit 0) and / or velocities (bit 1)
+ * into caller order, and a copy to pinned host memory on a stream of its own:
+ * the call returns at once and later steps run while the copy travels.  The
+ * last 4 snapshots stay readable by their ticket. */
+int parm_snapshot_begin(parm_system *sys, int what, int64_t *ticket);
+/* Waits for that snapshot and copies it out (natoms x ndim each; the box
+ * edges at the time of the snapshot into L[ndim]); NULL skips. */
+int parm_snapshot_read(parm_system *sys, int64_t ticket, double *x, double *v, double *L);
+/* Writes that snapshot's positions as one xyz frame, as the reference's
+ * drivers do: the atom count, the comment line, then per atom its element
+ * letter (elements[i], or 'C' when elements is NULL) and the ndim components
+ * of OriginBox::diff(0, x), tab-separated, printed like an ostream with the
+ * given precision (6 is the default; src/bin/packer.cpp:170 uses 24).
+ * append != 0 adds the frame to the file. */
+int parm_snapshot_write_xyz(parm_system *sys, int64_t ticket, const char *path, int append,
+                            const char *comment, const char *elements, int precision);
 
 /* ------------------------------------------------------ instrumentation */
 /* No reference counterpart (the reference has no timers); for benchmarks.
@@ -248,8 +372,24 @@ int parm_batch_damped_steps(parm_batch *b, double dt, double damping, int64_t ns
                             int use_graph);
 int parm_batch_verlet_steps(parm_batch *b, double dt, int64_t nsteps, int use_graph);
 /* NListed::energy and NListed::contacts per system (src/interaction.hpp:
- * 2125-2150, 2255-2291); nsys values each, NULL skips. */
+ * 2125-2150, 2153-2163); nsys values each, NULL skips. */
 int parm_batch_sums(parm_batch *b, double *U, double *contacts);
+/* Per system: NListed::energy (src/interaction.hpp:2153-2163), the virial
+ * sum r.f of NListed::pressure (:2247-2261), NListed::contacts and overlaps
+ * (:2125-2151).  nsys values each, NULL skips. */
+int parm_batch_pair_sums(parm_batch *b, double *U, double *W, double *contacts,
+                         double *overlaps);
+/* No reference counterpart (performance tuning): picks the lanes per atom of
+ * the batched force kernel now (otherwise the first run of >= 16 steps does). */
+int parm_batch_autotune(parm_batch *b);
+/* No reference counterpart (instrumentation).  While enabled, runs launch
+ * their steps eagerly (same kernels as the graph replay) and bracket each
+ * step's force launch and rebuild launch with HIP events; read returns the
+ * summed milliseconds and counts (reset != 0 clears them). */
+int parm_batch_profile_enable(parm_batch *b, int enable);
+int parm_batch_profile_read(parm_batch *b, int reset, double *force_ms,
+                            int64_t *force_launches, double *rebuild_ms,
+                            int64_t *rebuild_launches);
 /* NeighborList::which() and numpairs() per system (src/trackers.hpp:187-212). */
 int parm_batch_lists(parm_batch *b, uint32_t *which, int64_t *npairs);
 /* The pairs of system sysid's list as (first, last) atom indices within the

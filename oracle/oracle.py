@@ -21,6 +21,12 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 LJ_ATTRACT_REPULSE = 1
 REPULSION = 2
+LJ_CUT = 3           # NListed<EpsSigCutAtom, LennardJonesCutPair>
+LJ_CUT_II = 4        # NListed<IEpsISigCutAtom, LennardJonesCutPair>
+LJ_REPULSE = 5       # NListed<EpsSigAtom, LJRepulsePair>
+REPULSION_II = 6     # NListed<IEpsISigExpAtom, RepulsionPair>
+_EPS_ROWS = (LJ_ATTRACT_REPULSE, LJ_CUT_II, REPULSION_II)
+_SIG_ROWS = (LJ_CUT_II, REPULSION_II)
 
 _lib = None
 
@@ -52,7 +58,7 @@ def lib():
             fn = getattr(L, nm)
             fn.restype = C.c_int64
             fn.argtypes = [C.c_int, C.c_int64, _ip, _dp, _dp, _dp, C.c_double,
-                           C.c_int64, _ip, _ip]
+                           C.c_int64, _ip, _ip, C.c_int64, _ip, _ip]
         L.or_skin_check.restype = C.c_int
         L.or_skin_check.argtypes = [C.c_int, C.c_int64, _ip, _dp, _dp, C.c_double]
         L.or_forces.restype = None
@@ -66,6 +72,9 @@ def lib():
         L.or_sys_destroy.argtypes = [C.c_void_p]
         L.or_sys_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.or_sys_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.or_sys_ignore.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.or_sys_ignore_size.restype = C.c_int64
+        L.or_sys_ignore_size.argtypes = [C.c_void_p]
         L.or_sys_update_list.restype = C.c_int
         L.or_sys_update_list.argtypes = [C.c_void_p, C.c_int]
         L.or_sys_which.restype = C.c_uint32
@@ -88,6 +97,18 @@ def lib():
         L.or_sys_verlet.argtypes = [C.c_void_p, C.c_double, C.c_int64]
         L.or_damped_constants.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_sys_damped.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int64]
+        L.or_nlcg_create.restype = C.c_void_p
+        L.or_nlcg_create.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_int, C.c_double, C.c_int]
+        L.or_nlcg_destroy.argtypes = [C.c_void_p]
+        L.or_nlcg_set_limits.argtypes = [C.c_void_p] + [C.c_double] * 5
+        L.or_nlcg_set_forces.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.or_nlcg_reset.argtypes = [C.c_void_p, C.c_void_p]
+        L.or_nlcg_timestep.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.or_nlcg_descend.argtypes = [C.c_void_p, C.c_void_p]
+        L.or_nlcg_observables.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp]
+        L.or_nlcg_info.argtypes = [C.c_void_p, _dp]
+        L.or_sys_box.argtypes = [C.c_void_p, _dp]
         _lib = L
     return _lib
 
@@ -117,17 +138,21 @@ def vec_mod(dx: float, L: float) -> float:
 
 class PairTable:
     """Per-type parameters of one pair interaction, as the reference's per-atom
-    structs hold them (IEpsSigCutAtom or EpsSigExpAtom)."""
+    structs hold them.  eps (and sigma) are per type, or ntypes x ntypes rows
+    for the indexed atom types (IEpsSigCutAtom's epsilons; IEpsISigCutAtom's
+    and IEpsISigExpAtom's epsilons and sigmas).  third is sigcut or the
+    exponent per type (ignored by LJ_REPULSE)."""
 
     def __init__(self, kind, eps, sigma, third, types):
         self.kind = int(kind)
-        self.sigma = _f64(sigma).ravel()
-        nt = self.sigma.size
-        if self.kind == LJ_ATTRACT_REPULSE:
+        sig = _f64(sigma)
+        nt = int(round(sig.size ** 0.5)) if self.kind in _SIG_ROWS else sig.size
+        self.sigma = _f64(sig, (nt, nt)) if self.kind in _SIG_ROWS else sig.ravel()
+        if self.kind in _EPS_ROWS:
             self.eps = _f64(eps, (nt, nt))
         else:
             self.eps = _f64(eps).ravel()
-        self.third = _f64(third).ravel()
+        self.third = (np.zeros(nt) if third is None else _f64(third).ravel())
         self.types = _i32(types)
         self.ntypes = nt
 
@@ -144,20 +169,23 @@ def _members(natoms, member):
     return _i32(np.arange(natoms) if member is None else member)
 
 
-def nlist(x, L, diam, skin, member=None, brute=False):
-    """Reference-order pair list (first, last) as int32 arrays."""
+def nlist(x, L, diam, skin, member=None, brute=False, ignore=None):
+    """Reference-order pair list (first, last) as int32 arrays.  ignore: an
+    array of excluded (a, b) atom-index pairs (NeighborList::ignore)."""
     x = _f64(x)
     n, ndim = x.shape
     mem = _members(n, member)
     diam = _f64(diam).ravel()
     L = _f64(L).ravel()
+    ex = np.zeros((0, 2), np.int32) if ignore is None else _i32(ignore).reshape(-1, 2)
+    ea, eb = _i32(ex[:, 0]), _i32(ex[:, 1])
     fn = lib().or_nlist_brute if brute else lib().or_nlist_cells
     cap = max(16, 64 * len(mem))
     while True:
         first = np.empty(cap, np.int32)
         last = np.empty(cap, np.int32)
         npairs = fn(ndim, len(mem), _i(mem), _d(diam), _d(x), _d(L), float(skin), cap,
-                    _i(first), _i(last))
+                    _i(first), _i(last), len(ex), _i(ea), _i(eb))
         if npairs <= cap:
             return first[:npairs].copy(), last[:npairs].copy()
         cap = int(npairs)
@@ -235,6 +263,12 @@ class System:
         lib().or_sys_get_state(self.h, _d(x), _d(v), _d(a), _d(f))
         return x, v, a, f
 
+    def ignore(self, a, b):
+        lib().or_sys_ignore(self.h, int(a), int(b))
+
+    def ignore_size(self):
+        return int(lib().or_sys_ignore_size(self.h))
+
     def update_list(self, force=True):
         return bool(lib().or_sys_update_list(self.h, int(force)))
 
@@ -291,3 +325,55 @@ class System:
 
     def damped(self, dt, damping, nsteps=1):
         lib().or_sys_damped(self.h, float(dt), float(damping), int(nsteps))
+
+    def box(self):
+        L = np.zeros(self.ndim)
+        lib().or_sys_box(self.h, _d(L))
+        return L
+
+
+class NLCG:
+    """CollectionNLCG (or CollectionNLCGFixedL) on an oracle System; creating
+    it runs Collection::initialize on the system."""
+
+    INFO = ("alpha", "beta", "betaused", "dxsum", "alphavmax", "sec", "k", "vl", "fl",
+            "al", "Knew", "maxdV")
+
+    def __init__(self, system, dt, P0, kappa=10.0, kmax=1000, secmax=40, seceps=1e-20,
+                 fixed_L=False):
+        self.system = system
+        self.h = lib().or_nlcg_create(system.h, float(dt), float(P0), float(kappa),
+                                      float(kmax), int(secmax), float(seceps),
+                                      int(bool(fixed_L)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_nlcg_destroy(self.h)
+            self.h = None
+
+    def set_limits(self, alphamax=2.0, afrac=0.0, dxmax=100.0, stepmax=1e-3, maxdV=0.0):
+        lib().or_nlcg_set_limits(self.h, float(alphamax), float(afrac), float(dxmax),
+                                 float(stepmax), float(maxdV))
+
+    def set_forces(self, constraints_and_a=True, setV=True):
+        lib().or_nlcg_set_forces(self.system.h, self.h, int(bool(constraints_and_a)),
+                                 int(bool(setV)))
+
+    def reset(self):
+        lib().or_nlcg_reset(self.system.h, self.h)
+
+    def timestep(self, nsteps=1):
+        lib().or_nlcg_timestep(self.system.h, self.h, int(nsteps))
+
+    def descend(self):
+        lib().or_nlcg_descend(self.system.h, self.h)
+
+    def observables(self):
+        H, P, K = np.zeros(1), np.zeros(1), np.zeros(1)
+        lib().or_nlcg_observables(self.system.h, self.h, _d(H), _d(P), _d(K))
+        return dict(H=float(H[0]), P=float(P[0]), K=float(K[0]))
+
+    def info(self):
+        out = np.zeros(len(self.INFO))
+        lib().or_nlcg_info(self.h, _d(out))
+        return dict(zip(self.INFO, out.tolist()))

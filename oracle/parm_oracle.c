@@ -35,7 +35,43 @@ struct or_system {
     int ignorechanged;
     void *pairs;
     uint32_t pairs_which;
+    /* NeighborList::ignorepairs (src/trackers.hpp:41-128): sorted keys
+     * (hi << 32 | lo) of atom-index pairs, hi >= lo */
+    int64_t nexcl, excl_cap;
+    uint64_t *excl;
 };
+
+/* CollectionNLCG's members (src/collection.hpp:408-430). */
+struct or_nlcg {
+    int fixedL;
+    double dt, seceps;
+    unsigned secmax;
+    double kappa, alphamax, afrac, dxmax, stepmax, kmax, P0;
+    double Knew, k, vl, fl, al;
+    double alpha, beta, betaused, dxsum, alphavmax, maxdV;
+    unsigned sec;
+};
+
+static uint64_t excl_key(int32_t a, int32_t b) {
+    uint32_t hi = (uint32_t)(a > b ? a : b), lo = (uint32_t)(a > b ? b : a);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+/* PairList::has_pair (src/trackers.hpp:66-71): the unordered pair is stored. */
+static int or_excluded(int64_t n, const uint64_t *keys, int32_t a, int32_t b) {
+    if (n == 0) return 0;
+    uint64_t k = excl_key(a, b);
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (keys[mid] == k) return 1;
+        if (keys[mid] < k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return 0;
+}
 
 #define SUF(name) name##_2d
 #define NDIM 2
@@ -69,22 +105,52 @@ double or_pair(int ndim, int kind, int ntypes, const double *eps_tab,
     }
 }
 
+static int cmp_u64(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return (x > y) - (x < y);
+}
+
+/* Sorted, de-duplicated exclusion keys of nexcl (ea[k], eb[k]) pairs. */
+static int64_t excl_keys(int64_t nexcl, const int32_t *ea, const int32_t *eb,
+                         uint64_t **out) {
+    uint64_t *k = (uint64_t *)malloc(sizeof(uint64_t) * (nexcl > 0 ? nexcl : 1));
+    for (int64_t q = 0; q < nexcl; q++) k[q] = excl_key(ea[q], eb[q]);
+    qsort(k, (size_t)nexcl, sizeof(uint64_t), cmp_u64);
+    int64_t m = 0;
+    for (int64_t q = 0; q < nexcl; q++)
+        if (m == 0 || k[q] != k[m - 1]) k[m++] = k[q];
+    *out = k;
+    return m;
+}
+
 int64_t or_nlist_brute(int ndim, int64_t M, const int32_t *member,
                        const double *diam, const double *x, const double *L,
                        double skin, int64_t cap, int32_t *first,
-                       int32_t *last) {
-    return ndim == 3
-               ? nlist_brute_3d(M, member, diam, x, L, skin, cap, first, last)
-               : nlist_brute_2d(M, member, diam, x, L, skin, cap, first, last);
+                       int32_t *last, int64_t nexcl, const int32_t *ea,
+                       const int32_t *eb) {
+    uint64_t *k;
+    int64_t m = excl_keys(nexcl, ea, eb, &k);
+    int64_t r = ndim == 3 ? nlist_brute_3d(M, member, diam, x, L, skin, cap,
+                                           first, last, m, k)
+                          : nlist_brute_2d(M, member, diam, x, L, skin, cap,
+                                           first, last, m, k);
+    free(k);
+    return r;
 }
 
 int64_t or_nlist_cells(int ndim, int64_t M, const int32_t *member,
                        const double *diam, const double *x, const double *L,
                        double skin, int64_t cap, int32_t *first,
-                       int32_t *last) {
-    return ndim == 3
-               ? nlist_cells_3d(M, member, diam, x, L, skin, cap, first, last)
-               : nlist_cells_2d(M, member, diam, x, L, skin, cap, first, last);
+                       int32_t *last, int64_t nexcl, const int32_t *ea,
+                       const int32_t *eb) {
+    uint64_t *k;
+    int64_t m = excl_keys(nexcl, ea, eb, &k);
+    int64_t r = ndim == 3 ? nlist_cells_3d(M, member, diam, x, L, skin, cap,
+                                           first, last, m, k)
+                          : nlist_cells_2d(M, member, diam, x, L, skin, cap,
+                                           first, last, m, k);
+    free(k);
+    return r;
 }
 
 int or_skin_check(int ndim, int64_t M, const int32_t *member, const double *x,
@@ -135,10 +201,11 @@ or_system *or_sys_create(int ndim, int64_t natoms, const double *masses,
     for (int k = 0; k < ndim; k++) s->L[k] = L[k];
     s->kind = kind;
     s->ntypes = ntypes;
-    s->eps_tab = dup_d(eps_tab, kind == OR_LJ_ATTRACT_REPULSE
-                                    ? (int64_t)ntypes * ntypes
-                                    : ntypes);
-    s->sig_tab = dup_d(sig_tab, ntypes);
+    const int eps_rows = kind == OR_LJ_ATTRACT_REPULSE || kind == OR_LJ_CUT_II ||
+                         kind == OR_REPULSION_II;
+    const int sig_rows = kind == OR_LJ_CUT_II || kind == OR_REPULSION_II;
+    s->eps_tab = dup_d(eps_tab, eps_rows ? (int64_t)ntypes * ntypes : ntypes);
+    s->sig_tab = dup_d(sig_tab, sig_rows ? (int64_t)ntypes * ntypes : ntypes);
     s->p3_tab = dup_d(p3_tab, ntypes);
     s->type = (int32_t *)malloc(sizeof(int32_t) * (natoms > 0 ? natoms : 1));
     memcpy(s->type, type, sizeof(int32_t) * natoms);
@@ -176,8 +243,32 @@ void or_sys_destroy(or_system *s) {
     free(s->first);
     free(s->last);
     free(s->pairs);
+    free(s->excl);
     free(s);
 }
+
+/* NeighborList::ignore(a, b) (src/trackers.hpp:190-193): PairList::add_pair
+ * (:76-83) stores the unordered pair once; the next update rebuilds. */
+void or_sys_ignore(or_system *s, int32_t a, int32_t b) {
+    uint64_t k = excl_key(a, b);
+    int64_t pos = 0;
+    while (pos < s->nexcl && s->excl[pos] < k) pos++;
+    if (pos < s->nexcl && s->excl[pos] == k) {
+        s->ignorechanged = 1;
+        return;
+    }
+    if (s->nexcl == s->excl_cap) {
+        s->excl_cap = s->excl_cap ? 2 * s->excl_cap : 16;
+        s->excl = (uint64_t *)realloc(s->excl, sizeof(uint64_t) * s->excl_cap);
+    }
+    memmove(s->excl + pos + 1, s->excl + pos, sizeof(uint64_t) * (s->nexcl - pos));
+    s->excl[pos] = k;
+    s->nexcl++;
+    s->ignorechanged = 1;
+}
+
+/* NeighborList::ignore_size (src/trackers.hpp:204, PairList::size :117-123). */
+int64_t or_sys_ignore_size(const or_system *s) { return s->nexcl; }
 
 #define FOR_ATOMS(body3, body2) do {                             \
     if (s->ndim == 3) {                                          \
@@ -313,4 +404,101 @@ void or_sys_damped(or_system *s, double dt, double damping, int64_t nsteps) {
         sys_damped_3d(s, dt, c0, c1, c2, nsteps);
     else
         sys_damped_2d(s, dt, c0, c1, c2, nsteps);
+}
+
+/* ------------------------------------------------------------------ NLCG */
+or_nlcg *or_nlcg_create(or_system *s, double dt, double P0, double kappa,
+                        double kmax, int secmax, double seceps, int fixedL) {
+    /* Collection's constructor runs initialize() (src/collection.cpp:3-19)
+     * with Collection::set_forces; the NLCG members start as in
+     * src/collection.cpp:500-523 */
+    or_sys_initialize(s);
+    or_nlcg *c = (or_nlcg *)calloc(1, sizeof(or_nlcg));
+    c->fixedL = fixedL;
+    c->dt = dt;
+    c->P0 = P0;
+    c->kappa = kappa;
+    c->kmax = kmax;
+    c->secmax = (unsigned)secmax;
+    c->seceps = seceps;
+    c->alphamax = 2.0;
+    c->afrac = 0;
+    c->dxmax = 100;
+    c->stepmax = 1e-3;
+    return c;
+}
+
+void or_nlcg_destroy(or_nlcg *c) { free(c); }
+
+void or_nlcg_set_limits(or_nlcg *c, double alphamax, double afrac, double dxmax,
+                        double stepmax, double maxdV) {
+    c->alphamax = alphamax;
+    c->afrac = afrac;
+    c->dxmax = dxmax;
+    c->stepmax = stepmax;
+    c->maxdV = maxdV;
+}
+
+void or_nlcg_set_forces(or_system *s, or_nlcg *c, int cna, int setV) {
+    if (s->ndim == 3)
+        nlcg_set_forces_3d(s, c, cna, setV);
+    else
+        nlcg_set_forces_2d(s, c, cna, setV);
+}
+
+void or_nlcg_reset(or_system *s, or_nlcg *c) {
+    /* reset (src/collection.cpp:528-535) */
+    c->k = 0;
+    or_nlcg_set_forces(s, c, 1, 1);
+    c->vl = c->al;
+}
+
+void or_nlcg_timestep(or_system *s, or_nlcg *c, int64_t nsteps) {
+    for (int64_t q = 0; q < nsteps; q++) {
+        if (s->ndim == 3)
+            nlcg_timestep_3d(s, c);
+        else
+            nlcg_timestep_2d(s, c);
+    }
+}
+
+void or_nlcg_descend(or_system *s, or_nlcg *c) {
+    if (s->ndim == 3)
+        nlcg_descend_3d(s, c);
+    else
+        nlcg_descend_2d(s, c);
+}
+
+void or_nlcg_observables(or_system *s, or_nlcg *c, double *H, double *P, double *K) {
+    double U = 0, W = 0, V;
+    if (s->ndim == 3) {
+        sys_pair_sums_3d(s, &U, &W, NULL);
+        V = box_V_3d(s);
+        *K = nlcg_kinetic_3d(s, c);
+    } else {
+        sys_pair_sums_2d(s, &U, &W, NULL);
+        V = box_V_2d(s);
+        *K = nlcg_kinetic_2d(s, c);
+    }
+    *H = U + c->P0 * V;
+    *P = W / V / (double)s->ndim;
+}
+
+void or_nlcg_info(const or_nlcg *c, double *out) {
+    out[0] = c->alpha;
+    out[1] = c->beta;
+    out[2] = c->betaused;
+    out[3] = c->dxsum;
+    out[4] = c->alphavmax;
+    out[5] = c->sec;
+    out[6] = c->k;
+    out[7] = c->vl;
+    out[8] = c->fl;
+    out[9] = c->al;
+    out[10] = c->Knew;
+    out[11] = c->maxdV;
+}
+
+void or_sys_box(const or_system *s, double *L) {
+    for (int k = 0; k < s->ndim; k++) L[k] = s->L[k];
 }

@@ -28,6 +28,14 @@
  *       sig_tab : sigma per type;  p3_tab : sigcut per type
  *   OR_REPULSION           NListed<EpsSigExpAtom, RepulsionPair>
  *       eps_tab : eps per type; sig_tab : sigma per type; p3_tab : exponent
+ *   OR_LJ_CUT              NListed<EpsSigCutAtom, LennardJonesCutPair>
+ *       eps_tab : eps per type; sig_tab : sigma per type; p3_tab : sigcut
+ *   OR_LJ_CUT_II           NListed<IEpsISigCutAtom, LennardJonesCutPair>
+ *       eps_tab, sig_tab : ntypes x ntypes rows; p3_tab : sigcut per type
+ *   OR_LJ_REPULSE          NListed<EpsSigAtom, LJRepulsePair>
+ *       eps_tab : eps per type; sig_tab : sigma per type; p3_tab unused
+ *   OR_REPULSION_II        NListed<IEpsISigExpAtom, RepulsionPair>
+ *       eps_tab, sig_tab : ntypes x ntypes rows; p3_tab : exponent per type
  */
 #ifndef PARM_ORACLE_H
 #define PARM_ORACLE_H
@@ -38,7 +46,14 @@
 extern "C" {
 #endif
 
-enum { OR_LJ_ATTRACT_REPULSE = 1, OR_REPULSION = 2 };
+enum {
+    OR_LJ_ATTRACT_REPULSE = 1,
+    OR_REPULSION = 2,
+    OR_LJ_CUT = 3,      /* NListed<EpsSigCutAtom, LennardJonesCutPair>      */
+    OR_LJ_CUT_II = 4,   /* NListed<IEpsISigCutAtom, LennardJonesCutPair>    */
+    OR_LJ_REPULSE = 5,  /* NListed<EpsSigAtom, LJRepulsePair>               */
+    OR_REPULSION_II = 6 /* NListed<IEpsISigExpAtom, RepulsionPair>          */
+};
 
 /* remainder(dx, L): OriginBox::diff per component (src/box.hpp:68-78). */
 double or_vec_mod1(double dx, double L);
@@ -58,15 +73,18 @@ double or_pair(int ndim, int kind, int ntypes, const double *eps_tab,
 int64_t or_nlist_brute(int ndim, int64_t nmembers, const int32_t *member,
                        const double *diam, const double *x, const double *L,
                        double skin, int64_t cap, int32_t *first,
-                       int32_t *last);
+                       int32_t *last, int64_t nexcl, const int32_t *ea,
+                       const int32_t *eb);
 
 /* Same pair set and order as or_nlist_brute, in O(M) with a periodic cell
  * grid and the identical predicate.  Used to check the device list at sizes
- * where the O(M^2) loop is too slow. */
+ * where the O(M^2) loop is too slow.  Both skip the nexcl excluded pairs
+ * (ea[k], eb[k]) as NeighborList::ignore does (src/trackers.cpp:64). */
 int64_t or_nlist_cells(int ndim, int64_t nmembers, const int32_t *member,
                        const double *diam, const double *x, const double *L,
                        double skin, int64_t cap, int32_t *first,
-                       int32_t *last);
+                       int32_t *last, int64_t nexcl, const int32_t *ea,
+                       const int32_t *eb);
 
 /* The O(N) skin test of NeighborList::update_list(false)
  * (src/trackers.cpp:23-53): running top-2 of |x_i - lastloc_i| over members
@@ -108,6 +126,9 @@ void or_sys_set_state(or_system *s, const double *x, const double *v,
                       const double *a, const double *f);
 void or_sys_get_state(const or_system *s, double *x, double *v, double *a,
                       double *f);
+/* NeighborList::ignore(a, b) / ignore_size() (src/trackers.hpp:190-204). */
+void or_sys_ignore(or_system *s, int32_t a, int32_t b);
+int64_t or_sys_ignore_size(const or_system *s);
 /* NeighborList::update_list(force)  (src/trackers.cpp:19-85); returns 1 if
  * the list was rebuilt. */
 int or_sys_update_list(or_system *s, int force);
@@ -135,6 +156,25 @@ void or_sys_verlet(or_system *s, double dt, int64_t nsteps);
 void or_damped_constants(double dt, double damping, double *c0, double *c1,
                          double *c2);
 void or_sys_damped(or_system *s, double dt, double damping, int64_t nsteps);
+
+/* CollectionNLCG / CollectionNLCGFixedL on the system (src/collection.hpp:
+ * 400-488, src/collection.cpp:494-870).  create runs Collection::initialize. */
+typedef struct or_nlcg or_nlcg;
+or_nlcg *or_nlcg_create(or_system *s, double dt, double P0, double kappa,
+                        double kmax, int secmax, double seceps, int fixedL);
+void or_nlcg_destroy(or_nlcg *c);
+void or_nlcg_set_limits(or_nlcg *c, double alphamax, double afrac, double dxmax,
+                        double stepmax, double maxdV);
+void or_nlcg_set_forces(or_system *s, or_nlcg *c, int constraints_and_a, int setV);
+void or_nlcg_reset(or_system *s, or_nlcg *c);
+void or_nlcg_timestep(or_system *s, or_nlcg *c, int64_t nsteps);
+void or_nlcg_descend(or_system *s, or_nlcg *c);
+/* hamiltonian, pressure, kinetic_energy (src/collection.cpp:571-606) */
+void or_nlcg_observables(or_system *s, or_nlcg *c, double *H, double *P, double *K);
+/* alpha, beta, betaused, dxsum, alphavmax, sec, k, vl, fl, al, Knew, maxdV */
+void or_nlcg_info(const or_nlcg *c, double *out);
+/* OriginBox::box_shape() of the system's box */
+void or_sys_box(const or_system *s, double *L);
 
 #ifdef __cplusplus
 }

@@ -49,10 +49,49 @@ static inline double SUF(dot)(const double *r, const double *f) {
  *   eps = |eps|; else cut_energy = eps * (1 - cut^-6)^2
  *   (src/interaction.hpp:1256-1269).
  * RepulsionPair(a1, a2): eps = sqrt(e1 e2), sig = (s1 + s2) / 2.0,
- *   exponent = (n1 + n2) / 2.0  (src/interaction.hpp:1532-1537). */
+ *   exponent = (n1 + n2) / 2.0  (src/interaction.hpp:1532-1537); with
+ *   IEpsISigExpAtom eps = a1.epsilons[a2.indx], sig = a1.sigmas[a2.indx]
+ *   (:1537-1542, get_epsilon / get_sigma :1500-1513).
+ * LennardJonesCutPair(a1, a2): EpsSigCutAtom eps = sqrt(e1 e2),
+ *   sig = (s1 + s2) / 2; IEpsISigCutAtom eps / sig from the rows; cut =
+ *   max(c1, c2); LennardJonesCut's cut_energy = eps ((1 - 1/cut^6)^2 - 1)
+ *   (src/interaction.hpp:970-984, 249-253).
+ * LJRepulsePair(a1, a2): eps = sqrt(e1 e2), sig = (s1 + s2) / 2
+ *   (src/interaction.hpp:875-883). */
 static void SUF(pair_params)(int kind, int ntypes, const double *eps_tab,
                              const double *sig_tab, const double *p3_tab,
                              int t1, int t2, SUF(pairP) *p) {
+    const int64_t ix = (int64_t)t1 * ntypes + t2;
+    if (kind == OR_LJ_CUT || kind == OR_LJ_CUT_II) {
+        double c1 = p3_tab[t1], c2 = p3_tab[t2];
+        double cut = (c1 < c2) ? c2 : c1; /* std::max */
+        if (kind == OR_LJ_CUT) {
+            p->eps = sqrt(eps_tab[t1] * eps_tab[t2]);
+            p->sig = (sig_tab[t1] + sig_tab[t2]) / 2;
+        } else {
+            p->eps = eps_tab[ix];
+            p->sig = sig_tab[ix];
+        }
+        double rsix = pow(cut, 6);
+        double mid = (1 - 1 / rsix);
+        p->p3 = cut;
+        p->p4 = p->eps * (mid * mid - 1);
+        return;
+    }
+    if (kind == OR_LJ_REPULSE) {
+        p->eps = sqrt(eps_tab[t1] * eps_tab[t2]);
+        p->sig = (sig_tab[t1] + sig_tab[t2]) / 2;
+        p->p3 = 1;
+        p->p4 = 0;
+        return;
+    }
+    if (kind == OR_REPULSION_II) {
+        p->eps = eps_tab[ix];
+        p->sig = sig_tab[ix];
+        p->p3 = (p3_tab[t1] + p3_tab[t2]) / 2.0;
+        p->p4 = 0;
+        return;
+    }
     if (kind == OR_LJ_ATTRACT_REPULSE) {
         double eps = eps_tab[(int64_t)t1 * ntypes + t2];
         double sig = (sig_tab[t1] + sig_tab[t2]) / 2.0;
@@ -81,12 +120,27 @@ static void SUF(pair_params)(int kind, int ntypes, const double *eps_tab,
 
 /* Force on atom1 (into f) for displacement rij = diff(x1, x2); returns the
  * pair energy.  LJAttractRepulsePair::forces / energy
- * (src/interaction.hpp:1270-1298) and RepulsionPair::forces / energy
- * (src/interaction.hpp:1538-1556). */
+ * (src/interaction.hpp:1270-1298), RepulsionPair::forces / energy
+ * (src/interaction.hpp:1544-1556), LennardJonesCut::forces / energy
+ * (src/interaction.hpp:254-272) and LJRepulsive::forces / energy
+ * (src/interaction.hpp:126-147). */
 static inline double SUF(pair_eval)(int kind, const SUF(pairP) *p,
                                     const double *rij, double *f) {
     for (int k = 0; k < NDIM; k++) f[k] = 0;
     double dsq = SUF(sqnorm)(rij);
+    if (kind == OR_LJ_CUT || kind == OR_LJ_CUT_II || kind == OR_LJ_REPULSE) {
+        /* LennardJonesCut (cut = p3) and LJRepulsive (cut 1, no shift) share
+         * their arithmetic: rsix = (r^2/sig^2)^3, 12 eps / rsix (1/rsix - 1) */
+        double rsq = dsq / (p->sig * p->sig);
+        if (rsq > (p->p3 * p->p3)) return 0;
+        double rsix = rsq * rsq * rsq;
+        double fmagTimesR = 12 * p->eps / rsix * (1 / rsix - 1);
+        double s = fmagTimesR / dsq;
+        for (int k = 0; k < NDIM; k++) f[k] = rij[k] * s;
+        double mid = (1 - 1 / rsix);
+        if (kind == OR_LJ_REPULSE) return p->eps * (mid * mid);
+        return p->eps * (mid * mid - 1) - p->p4;
+    }
     if (kind == OR_LJ_ATTRACT_REPULSE) {
         double rsq = dsq / (p->sig * p->sig);
         if (rsq > p->p3 * p->p3) return 0;
@@ -120,12 +174,14 @@ static inline int SUF(in_list)(const double *xi, const double *xj,
 static int64_t SUF(nlist_brute)(int64_t M, const int32_t *member,
                                 const double *diam, const double *x,
                                 const double *L, double skin, int64_t cap,
-                                int32_t *first, int32_t *last) {
+                                int32_t *first, int32_t *last, int64_t nexcl,
+                                const uint64_t *excl) {
     int64_t np = 0;
     for (int64_t i = 0; i < M; i++) {
         const double *xi = x + (int64_t)member[i] * NDIM;
         for (int64_t j = 0; j < i; j++) {
             const double *xj = x + (int64_t)member[j] * NDIM;
+            if (or_excluded(nexcl, excl, member[i], member[j])) continue;
             if (SUF(in_list)(xi, xj, L, diam[i], diam[j], skin)) {
                 if (np < cap) {
                     first[np] = member[i];
@@ -148,7 +204,8 @@ static int SUF(cmp_i32)(const void *a, const void *b) {
 static int64_t SUF(nlist_cells)(int64_t M, const int32_t *member,
                                 const double *diam, const double *x,
                                 const double *L, double skin, int64_t cap,
-                                int32_t *first, int32_t *last) {
+                                int32_t *first, int32_t *last, int64_t nexcl,
+                                const uint64_t *excl) {
     if (M <= 0) return 0;
     double dmax = 0;
     for (int64_t i = 0; i < M; i++)
@@ -214,6 +271,7 @@ static int64_t SUF(nlist_cells)(int64_t M, const int32_t *member,
             for (int64_t t = head[cc]; t < head[cc + 1]; t++) {
                 int32_t j = order[t];
                 if (j >= i) continue;
+                if (or_excluded(nexcl, excl, member[i], member[j])) continue;
                 const double *xj = x + (int64_t)member[j] * NDIM;
                 if (SUF(in_list)(xi, xj, L, diam[i], diam[j], skin))
                     cand[ncand++] = j;
@@ -322,10 +380,12 @@ static void SUF(sys_rebuild)(or_system *s) {
     for (;;) {
         if (s->brute)
             np = SUF(nlist_brute)(s->nmembers, s->member, s->diam, xs, s->L,
-                                  s->skin, s->pcap, s->first, s->last);
+                                  s->skin, s->pcap, s->first, s->last,
+                                  s->nexcl, s->excl);
         else
             np = SUF(nlist_cells)(s->nmembers, s->member, s->diam, xs, s->L,
-                                  s->skin, s->pcap, s->first, s->last);
+                                  s->skin, s->pcap, s->first, s->last,
+                                  s->nexcl, s->excl);
         if (np <= s->pcap) break;
         s->pcap = np + np / 4 + 16;
         s->first = (int32_t *)realloc(s->first, sizeof(int32_t) * s->pcap);
@@ -513,4 +573,217 @@ static void SUF(sys_damped)(or_system *s, double dt, double c0, double c1,
         }
         SUF(sys_update_list)(s, 0);
     }
+}
+
+/* ------------------------------------------------------------------ NLCG */
+/* CollectionNLCG / CollectionNLCGFixedL (src/collection.cpp:494-870), with
+ * the system's one NListed interaction and its NeighborList as the only
+ * tracker.  Atom loops in atom order, sums in that order. */
+
+/* Collection::set_forces_get_pressure(false) (src/collection.cpp:181-208)
+ * with NListed::set_forces_get_pressure (src/interaction.hpp:2231-2245). */
+static double SUF(sys_forces_get_pressure)(or_system *s) {
+    SUF(atom) *A = (SUF(atom) *)s->atoms;
+    for (int64_t i = 0; i < s->natoms; i++)
+        for (int k = 0; k < NDIM; k++) A[i].f[k] = 0;
+    SUF(sys_update_pairs)(s);
+    const SUF(pairP) *P = (const SUF(pairP) *)s->pairs;
+    double p = 0;
+    for (int64_t q = 0; q < s->npairs; q++) {
+        double r[NDIM], fp[NDIM];
+        SUF(atom) *a1 = &A[P[q].i], *a2 = &A[P[q].j];
+        SUF(diff)(a1->x, a2->x, s->L, r);
+        SUF(pair_eval)(s->kind, &P[q], r, fp);
+        for (int k = 0; k < NDIM; k++) {
+            a1->f[k] += fp[k];
+            a2->f[k] -= fp[k];
+        }
+        p += SUF(dot)(r, fp);
+    }
+    for (int64_t i = 0; i < s->natoms; i++)
+        if (SUF(is_fixed)(A[i].m))
+            for (int k = 0; k < NDIM; k++) A[i].a[k] = 0;
+    return p;
+}
+
+static double SUF(box_V)(const or_system *s) {
+    double V = 1;
+    for (int k = 0; k < NDIM; k++) V *= s->L[k];
+    return V;
+}
+
+/* get_length_squared (src/collection.cpp:627-635) */
+static double SUF(nlcg_l2)(const or_system *s) {
+#if NDIM == 3
+    return pow(SUF(box_V)(s), 2.0 / 3.0);
+#else
+    return SUF(box_V)(s);
+#endif
+}
+
+/* fdotf / fdota / fdotv / vdotv (src/collection.cpp:637-683):
+ * which = 0 f.f, 1 f.a, 2 f.v, 3 v.v over atoms that are not fixed. */
+static double SUF(nlcg_dot)(const or_system *s, const or_nlcg *c, int which) {
+    const SUF(atom) *A = (const SUF(atom) *)s->atoms;
+    double r = 0;
+    for (int64_t i = 0; i < s->natoms; i++) {
+        if (SUF(is_fixed)(A[i].m)) continue;
+        const double *p = which == 3 ? A[i].v : A[i].f;
+        const double *q = which == 0 ? A[i].f : which == 1 ? A[i].a : A[i].v;
+        r += SUF(dot)(p, q);
+    }
+    double sc = which == 0 ? c->fl * c->fl
+                : which == 1 ? c->fl * c->al
+                : which == 2 ? c->fl * c->vl
+                             : c->vl * c->vl;
+    return r / SUF(nlcg_l2)(s) + sc;
+}
+
+/* stepx (src/collection.cpp:608-625; FixedL :855-870) */
+static void SUF(nlcg_stepx)(or_system *s, or_nlcg *c, double dx) {
+    SUF(atom) *A = (SUF(atom) *)s->atoms;
+    double Lfac = c->fixedL ? exp(dx * c->vl / c->kappa)
+                            : exp(dx * c->vl / (c->kappa * NDIM));
+    for (int64_t i = 0; i < s->natoms; i++) {
+        if (c->fixedL)
+            A[i].x[0] *= Lfac;
+        else
+            for (int k = 0; k < NDIM; k++) A[i].x[k] *= Lfac;
+        for (int k = 0; k < NDIM; k++) A[i].x[k] += A[i].v[k] * dx;
+    }
+    if (c->fixedL)
+        s->L[0] *= Lfac;
+    else
+        for (int k = 0; k < NDIM; k++) s->L[k] *= Lfac;
+}
+
+/* CollectionNLCG::set_forces(constraints_and_a, setV) (:537-569) */
+static void SUF(nlcg_set_forces)(or_system *s, or_nlcg *c, int cna, int setV) {
+    double V = SUF(box_V)(s);
+    if (setV) {
+        double interacP = SUF(sys_forces_get_pressure)(s);
+        c->fl = ((interacP / NDIM) - (c->P0 * V)) / c->kappa;
+        if (cna) {
+            c->al = c->fl;
+            c->vl = c->fl;
+        }
+    } else {
+        SUF(sys_set_forces)(s, 0);
+    }
+    if (cna) {
+        SUF(atom) *A = (SUF(atom) *)s->atoms;
+        for (int64_t i = 0; i < s->natoms; i++) {
+            if (SUF(is_fixed)(A[i].m)) {
+                for (int k = 0; k < NDIM; k++) A[i].v[k] = A[i].a[k] = 0;
+                continue;
+            }
+            for (int k = 0; k < NDIM; k++) A[i].a[k] = A[i].v[k] = A[i].f[k];
+        }
+        c->Knew = SUF(nlcg_dot)(s, c, 1);
+    }
+}
+
+/* CollectionNLCG::timestep (src/collection.cpp:685-835) */
+static void SUF(nlcg_timestep)(or_system *s, or_nlcg *c) {
+    SUF(nlcg_stepx)(s, c, c->dt);
+    SUF(nlcg_set_forces)(s, c, 0, 1);
+    double eta0 = -SUF(nlcg_dot)(s, c, 2);
+    double eta;
+    SUF(nlcg_stepx)(s, c, -c->dt);
+    SUF(sys_update_list)(s, 0);
+    c->alpha = -c->dt;
+    SUF(nlcg_set_forces)(s, c, 0, 1);
+    c->dxsum = 0;
+    double vdv = SUF(nlcg_dot)(s, c, 3);
+    for (c->sec = 0; c->sec < c->secmax; c->sec++) {
+        eta = -SUF(nlcg_dot)(s, c, 2);
+        double alphafac = -eta / fabs(eta0 - eta);
+        if (fabs(eta0 - eta) <= 1e-12 * fabs(eta)) {
+            alphafac = c->alphamax > 0 ? c->alphamax : 1.1;
+            c->sec = c->sec > 0 ? c->sec * 2 - 1 : 1;
+        }
+        if (c->alphamax > 0 && alphafac > c->alphamax) alphafac = c->alphamax;
+        if (c->alphamax > 0 && alphafac < -c->alphamax) alphafac = -c->alphamax;
+        c->alpha = fabs(c->alpha) * alphafac;
+        double newdxsum = fabs(c->dxsum + c->alpha);
+        if (c->dxmax > 0 && newdxsum > c->dxmax) {
+            c->k = 0;
+            break;
+        }
+        double dVoverV = expm1(fabs(c->dxsum + c->alpha) * c->vl / (c->kappa * NDIM));
+        if (c->maxdV > 0 && dVoverV > c->maxdV) {
+            c->k = 0;
+            break;
+        }
+        c->dxsum += c->alpha;
+        SUF(nlcg_stepx)(s, c, c->alpha);
+        SUF(nlcg_set_forces)(s, c, 0, 1);
+        eta0 = eta;
+        if (c->alpha * c->alpha * vdv < c->seceps * c->seceps) break;
+        if ((c->sec > 1) && (c->afrac > 0) && (fabs(c->alpha) < fabs(c->dxsum)) &&
+            (fabs(c->alpha) / fabs(c->dxsum) < c->afrac))
+            break;
+        if (c->stepmax > 0 && c->dxsum * c->dxsum * vdv > c->stepmax * c->stepmax) {
+            c->k = 0;
+            break;
+        }
+    }
+    c->alphavmax = sqrt(c->alpha * c->alpha * vdv);
+    double Kold = c->Knew;
+    double Kmid = SUF(nlcg_dot)(s, c, 1);
+    SUF(atom) *A = (SUF(atom) *)s->atoms;
+    for (int64_t i = 0; i < s->natoms; i++) {
+        if (SUF(is_fixed)(A[i].m)) {
+            for (int k = 0; k < NDIM; k++) A[i].a[k] = 0;
+            continue;
+        }
+        for (int k = 0; k < NDIM; k++) A[i].a[k] = A[i].f[k];
+    }
+    c->al = c->fl;
+    c->Knew = SUF(nlcg_dot)(s, c, 1);
+    c->beta = (c->Knew - Kmid) / Kold;
+    c->betaused = c->beta;
+    c->k++;
+    if (c->k >= c->kmax || isinf(c->betaused) || isnan(c->betaused) || c->betaused <= 0) {
+        c->k = 0;
+        c->betaused = 0;
+    } else if (c->betaused > 1) {
+        c->betaused = 1;
+    }
+    for (int64_t i = 0; i < s->natoms; i++) {
+        if (SUF(is_fixed)(A[i].m)) {
+            for (int k = 0; k < NDIM; k++) A[i].v[k] = 0;
+            continue;
+        }
+        for (int k = 0; k < NDIM; k++) A[i].v[k] = A[i].a[k] + A[i].v[k] * c->betaused;
+    }
+    c->vl = c->al + c->betaused * c->vl;
+}
+
+/* descend (src/collection.cpp:837-853) */
+static void SUF(nlcg_descend)(or_system *s, or_nlcg *c) {
+    SUF(nlcg_set_forces)(s, c, 0, 1);
+    SUF(atom) *A = (SUF(atom) *)s->atoms;
+    for (int64_t i = 0; i < s->natoms; i++) {
+        if (SUF(is_fixed)(A[i].m)) continue;
+        for (int k = 0; k < NDIM; k++) A[i].v[k] = A[i].a[k] = A[i].f[k];
+    }
+    c->al = c->fl;
+    c->vl = c->fl;
+    SUF(nlcg_stepx)(s, c, c->dt);
+    SUF(sys_update_list)(s, 0);
+}
+
+/* kinetic_energy (src/collection.cpp:575-590): masses ignored */
+static double SUF(nlcg_kinetic)(const or_system *s, const or_nlcg *c) {
+    const SUF(atom) *A = (const SUF(atom) *)s->atoms;
+    double Lfac = exp(c->vl / (c->kappa * NDIM));
+    double E = 0;
+    for (int64_t i = 0; i < s->natoms; i++) {
+        if (SUF(is_fixed)(A[i].m)) continue;
+        double w[NDIM];
+        for (int k = 0; k < NDIM; k++) w[k] = A[i].v[k] + (A[i].x[k] * Lfac);
+        E += SUF(dot)(w, w);
+    }
+    return E / 2.0;
 }

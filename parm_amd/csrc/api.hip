@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 #include "../../include/parm_hip.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "nlcg.hpp"
 #include "pairtable.hpp"
 #include "tiles.hpp"
 
@@ -138,6 +140,14 @@ struct parm_system {
     int cstart_cap = 0;
     unsigned long long *stats = nullptr, *bpart = nullptr;
 
+    // NeighborList::ignore: excluded pairs as sorted keys (hi << 32 | lo) of
+    // atom indices, and their device CSR (both directions, over original
+    // atom indices) uploaded before the next build when changed
+    std::vector<uint64_t> excl;
+    bool excl_dirty = false;
+    int32_t *ex_off = nullptr, *ex_part = nullptr;
+    int64_t ex_part_cap = 0, ex_npart = 0;
+
     // pair interaction
     int pair_kind = kPairNone;
     int ntypes = 0;
@@ -152,6 +162,21 @@ struct parm_system {
     double *hout = nullptr;         // pinned mirror of out
     hipEvent_t ev_flag = nullptr;
     hipEvent_t ev_ring[kFlagRing] = {};
+
+    // asynchronous snapshots for output: a ring of device staging buffers
+    // and pinned host copies, filled on the system's stream and copied to the
+    // host on a stream of their own
+    struct Snap {
+        double *dev = nullptr, *host = nullptr;
+        hipEvent_t ready = nullptr, done = nullptr;
+        int64_t ticket = -1;
+        int what = 0;
+        double L[3] = {0, 0, 0};
+    };
+    static constexpr int kSnapRing = 4;
+    Snap snap[kSnapRing];
+    hipStream_t out_stream = nullptr;
+    int64_t snap_next = 0;
 
     // instrumentation: event pairs around step-kernel launches and rebuilds
     bool prof = false;
@@ -202,6 +227,14 @@ struct parm_system {
         P.harmonic = harmonic;
         P.tab = tab;
         return P;
+    }
+    ExclView exview() const {
+        ExclView E{};
+        E.orig = orig;
+        E.off = ex_off;
+        E.part = ex_part;
+        E.any = ex_npart > 0 ? 1 : 0;
+        return E;
     }
     ReduceBuf redbuf(int flag_index) const {
         ReduceBuf R{};
@@ -397,13 +430,14 @@ int build_list(parm_system *s, const void *at) {
                                     s->cstart, s->grid, s->box, s->skin, s->dmax, s->tpa,
                                     s->cap, s->npad, !s->tiled, s->nbr, s->ncount, s->lcount,
                                     s->xref, s->bpart, s->stats, s->dflag + kSlotBuild,
-                                    nullptr, s->dmin == s->dmax, s->hrep_dev, st));
+                                    nullptr, s->dmin == s->dmax, s->exview(), s->hrep_dev,
+                                    st));
     } else {
         PARM_HIP(launch_build_list(s->ndim, s->n, at, s->xwrap, s->rank, s->diam, s->type,
                                    s->keys, s->cstart, s->grid, s->box, s->skin, s->dmax,
                                    s->type_bits, s->tpa, s->cap, s->npad, s->nbr, s->ncount,
                                    s->lcount, s->xref, s->bpart, s->stats, s->dflag + kSlotBuild,
-                                   s->hrep_dev, st));
+                                   s->exview(), s->hrep_dev, st));
     }
     return PARM_OK;
 }
@@ -464,7 +498,7 @@ int build_speculative(parm_system *s) {
                                 s->box, s->skin, s->dmax, s->tpa, s->cap, s->npad, !tiled,
                                 s->nbr, s->ncount, s->lcount, s->xref, s->bpart, s->stats,
                                 s->dflag + kSlotBuild, s->tinfo, s->dmin == s->dmax,
-                                s->hrep_dev, s->stream));
+                                s->exview(), s->hrep_dev, s->stream));
     BuildReport r{};
     PARM_CHECK(read_report(s, &r));
     accept_tiles(s, r.ntiles, r.smax, r.bad);
@@ -479,10 +513,53 @@ int build_speculative(parm_system *s) {
     return build_checked(s, s->pos);
 }
 
+// The exclusion CSR of s->excl on the device: for every atom (original
+// index) its excluded partners, sorted; a pair of an atom with itself never
+// reaches the predicate and is left out.
+int upload_exclusions(parm_system *s) {
+    if (!s->excl_dirty) return PARM_OK;
+    const int n = s->n;
+    std::vector<int32_t> off((size_t)n + 1, 0);
+    for (uint64_t k : s->excl) {
+        const int32_t hi = (int32_t)(k >> 32), lo = (int32_t)(k & 0xffffffffu);
+        if (hi == lo) continue;
+        off[(size_t)hi + 1]++;
+        off[(size_t)lo + 1]++;
+    }
+    for (int i = 0; i < n; i++) off[(size_t)i + 1] += off[i];
+    const int64_t np = off[n];
+    std::vector<int32_t> part((size_t)std::max<int64_t>(np, 1));
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (uint64_t k : s->excl) {
+        const int32_t hi = (int32_t)(k >> 32), lo = (int32_t)(k & 0xffffffffu);
+        if (hi == lo) continue;
+        part[fill[hi]++] = lo;
+        part[fill[lo]++] = hi;
+    }
+    for (int i = 0; i < n; i++) std::sort(part.begin() + off[i], part.begin() + off[i + 1]);
+    if (!s->ex_off) PARM_CHECK(dalloc(&s->ex_off, (size_t)n + 1));
+    if (np > s->ex_part_cap) {
+        dfree(s->ex_part);
+        s->ex_part_cap = np + np / 2 + 16;
+        PARM_CHECK(dalloc(&s->ex_part, (size_t)s->ex_part_cap));
+    }
+    if (np > 0) {
+        PARM_HIP(hipMemcpyAsync(s->ex_off, off.data(), sizeof(int32_t) * off.size(),
+                                hipMemcpyHostToDevice, s->stream));
+        PARM_HIP(hipMemcpyAsync(s->ex_part, part.data(), sizeof(int32_t) * np,
+                                hipMemcpyHostToDevice, s->stream));
+        PARM_HIP(hipStreamSynchronize(s->stream));
+    }
+    s->ex_npart = np;
+    s->excl_dirty = false;
+    return PARM_OK;
+}
+
 // NeighborList::update_list's rebuild branch (src/trackers.cpp:55-69):
 // lastlocs = x, then the full pair list from the predicate.
 int rebuild(parm_system *s, bool keep_af = true) {
     if (!s->list_cfg) return fail(PARM_E_STATE, "neighbour list not configured");
+    PARM_CHECK(upload_exclusions(s));
     hipStream_t st = s->stream;
     const int n = s->n;
     make_grid(s);
@@ -835,12 +912,24 @@ void destroy(parm_system *s) {
     dfree(s->stats);
     dfree(s->bpart);
     dfree(s->tab);
+    dfree(s->ex_off);
+    dfree(s->ex_part);
     dfree(s->part);
     dfree(s->out);
     dfree(s->dflag);
     if (s->hflag) (void)hipHostFree(s->hflag);
     if (s->hrep) (void)hipHostFree(s->hrep);
     if (s->hout) (void)hipHostFree(s->hout);
+    for (parm_system::Snap &q : s->snap) {
+        dfree(q.dev);
+        if (q.host) (void)hipHostFree(q.host);
+        if (q.ready) (void)hipEventDestroy(q.ready);
+        if (q.done) (void)hipEventDestroy(q.done);
+    }
+    if (s->out_stream) {
+        (void)hipStreamSynchronize(s->out_stream);
+        (void)hipStreamDestroy(s->out_stream);
+    }
     if (s->ev_flag) (void)hipEventDestroy(s->ev_flag);
     for (hipEvent_t &e : s->ev_ring)
         if (e) (void)hipEventDestroy(e);
@@ -860,18 +949,38 @@ __global__ void k_fill_i32(int n, int32_t *p, int32_t v) {
 
 int relayout(parm_system *s);
 
-int build_pair_table(parm_system *s, int kind, int ntypes, const double *eps,
+// OriginBox(Vec) / resize_to (src/box.cpp:17-20): edges L[ndim] > 0.
+int apply_box(parm_system *s, const double *L) {
+    for (int k = 0; k < s->ndim; k++)
+        if (!(L[k] > 0) || !std::isfinite(L[k]))
+            return fail(PARM_E_INVALID, "box edges must be finite and > 0");
+    for (int k = 0; k < 3; k++) {
+        double Lk = k < s->ndim ? L[k] : 1.0;
+        s->box.L[k] = Lk;
+        s->box.invL[k] = 1.0 / Lk;
+        s->box.halfL[k] = Lk / 2;
+    }
+    s->box_set = true;
+    return PARM_OK;
+}
+
+int build_pair_table(parm_system *s, int form, int ntypes, const double *eps,
                      const double *sigma, const double *third, const int32_t *atype) {
     if (ntypes < 1 || ntypes > 64)
         return fail(PARM_E_UNSUPPORTED, "ntypes must be in [1, 64]");
-    if (!eps || !sigma || !third || (!atype && s->n > 0))
+    if (!eps || !sigma || (!third && form != kFormLJRepulse) || (!atype && s->n > 0))
         return fail(PARM_E_INVALID, "null parameter array");
     for (int i = 0; i < s->n; i++)
         if (atype[i] < 0 || atype[i] >= ntypes)
             return fail(PARM_E_INVALID, "atom type out of range at atom " + std::to_string(i));
+    const std::string asym = check_rows_symmetric(form, ntypes, eps, sigma);
+    if (!asym.empty())
+        return fail(PARM_E_UNSUPPORTED,
+                    asym + ": the device evaluates each pair from both of its atoms");
+    const int kind = form_kernel_kind(form);
     std::vector<PairParam> tab;
     bool harm = true;
-    make_pair_table(kind, ntypes, eps, sigma, third, tab, harm);
+    make_pair_table(form, ntypes, eps, sigma, third, tab, harm);
     PARM_CHECK(set_device(s));
     dfree(s->tab);
     PARM_CHECK(dalloc(&s->tab, tab.size()));
@@ -1052,16 +1161,13 @@ int parm_system_info(const parm_system *s, int *ndim, int *precision, int64_t *n
 int parm_set_box(parm_system *s, const double *L) {
     PARM_CHECK(check_sys(s));
     if (!L) return fail(PARM_E_INVALID, "L is null");
-    for (int k = 0; k < s->ndim; k++)
-        if (!(L[k] > 0) || !std::isfinite(L[k]))
-            return fail(PARM_E_INVALID, "box edges must be finite and > 0");
-    for (int k = 0; k < 3; k++) {
-        double Lk = k < s->ndim ? L[k] : 1.0;
-        s->box.L[k] = Lk;
-        s->box.invL[k] = 1.0 / Lk;
-        s->box.halfL[k] = Lk / 2;
-    }
-    s->box_set = true;
+    return apply_box(s, L);
+}
+
+int parm_get_box(parm_system *s, double *L) {
+    PARM_CHECK(check_sys(s));
+    if (!L) return fail(PARM_E_INVALID, "L is null");
+    for (int k = 0; k < s->ndim; k++) L[k] = s->box.L[k];
     return PARM_OK;
 }
 
@@ -1130,6 +1236,9 @@ int parm_nlist_set(parm_system *s, double skin, int64_t nmembers, const int32_t 
     s->nmem = nmembers;
     s->list_cfg = true;
     s->ignorechanged = true;
+    // a new NeighborList has an empty PairList
+    s->excl.clear();
+    s->excl_dirty = true;
     s->built = false;
     s->npairs = 0;
     free_list(s);
@@ -1141,6 +1250,38 @@ int parm_nlist_update(parm_system *s, int force, int *rebuilt) {
     if (!s->box_set) return fail(PARM_E_STATE, "box not set (parm_set_box)");
     PARM_CHECK(set_device(s));
     return update_list(s, force != 0, rebuilt);
+}
+
+int parm_nlist_ignore(parm_system *s, int64_t npairs, const int32_t *a, const int32_t *b) {
+    PARM_CHECK(check_sys(s));
+    if (!s->list_cfg) return fail(PARM_E_STATE, "neighbour list not set (parm_nlist_set)");
+    if (npairs < 0 || (npairs > 0 && (!a || !b)))
+        return fail(PARM_E_INVALID, "bad exclusion pair arrays");
+    std::vector<uint64_t> add;
+    add.reserve((size_t)npairs);
+    for (int64_t k = 0; k < npairs; k++) {
+        if (a[k] < 0 || a[k] >= s->n || b[k] < 0 || b[k] >= s->n)
+            return fail(PARM_E_INVALID, "exclusion atom index out of range");
+        const uint32_t hi = (uint32_t)std::max(a[k], b[k]), lo = (uint32_t)std::min(a[k], b[k]);
+        add.push_back(((uint64_t)hi << 32) | lo);
+    }
+    std::sort(add.begin(), add.end());
+    std::vector<uint64_t> merged;
+    merged.reserve(s->excl.size() + add.size());
+    std::merge(s->excl.begin(), s->excl.end(), add.begin(), add.end(),
+               std::back_inserter(merged));
+    merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
+    s->excl.swap(merged);
+    s->excl_dirty = true;
+    // NeighborList::ignore sets ignorechanged: the next update rebuilds
+    s->ignorechanged = true;
+    return PARM_OK;
+}
+
+int parm_nlist_ignore_size(parm_system *s, int64_t *n) {
+    PARM_CHECK(check_sys(s));
+    if (n) *n = (int64_t)s->excl.size();
+    return PARM_OK;
 }
 
 int parm_nlist_which(parm_system *s, uint32_t *which) {
@@ -1231,14 +1372,43 @@ int parm_pair_lj_attract_repulse(parm_system *s, int ntypes, const double *eps_r
                                  const double *sigma, const double *sigcut,
                                  const int32_t *atom_type) {
     PARM_CHECK(check_sys(s));
-    return build_pair_table(s, kPairLJ, ntypes, eps_rows, sigma, sigcut, atom_type);
+    return build_pair_table(s, kFormLJAttractRepulse, ntypes, eps_rows, sigma, sigcut,
+                            atom_type);
 }
 
 int parm_pair_repulsion(parm_system *s, int ntypes, const double *eps,
                         const double *sigma, const double *exponent,
                         const int32_t *atom_type) {
     PARM_CHECK(check_sys(s));
-    return build_pair_table(s, kPairRepulsion, ntypes, eps, sigma, exponent, atom_type);
+    return build_pair_table(s, kFormRepulsion, ntypes, eps, sigma, exponent, atom_type);
+}
+
+int parm_pair_lj_cut(parm_system *s, int ntypes, const double *eps, const double *sigma,
+                     const double *sigcut, const int32_t *atom_type) {
+    PARM_CHECK(check_sys(s));
+    return build_pair_table(s, kFormLJCut, ntypes, eps, sigma, sigcut, atom_type);
+}
+
+int parm_pair_lj_cut_indexed(parm_system *s, int ntypes, const double *eps_rows,
+                             const double *sigma_rows, const double *sigcut,
+                             const int32_t *atom_type) {
+    PARM_CHECK(check_sys(s));
+    return build_pair_table(s, kFormLJCutIndexed, ntypes, eps_rows, sigma_rows, sigcut,
+                            atom_type);
+}
+
+int parm_pair_lj_repulse(parm_system *s, int ntypes, const double *eps, const double *sigma,
+                         const int32_t *atom_type) {
+    PARM_CHECK(check_sys(s));
+    return build_pair_table(s, kFormLJRepulse, ntypes, eps, sigma, nullptr, atom_type);
+}
+
+int parm_pair_repulsion_indexed(parm_system *s, int ntypes, const double *eps_rows,
+                                const double *sigma_rows, const double *exponent,
+                                const int32_t *atom_type) {
+    PARM_CHECK(check_sys(s));
+    return build_pair_table(s, kFormRepulsionIndexed, ntypes, eps_rows, sigma_rows, exponent,
+                            atom_type);
 }
 
 int parm_set_forces(parm_system *s, int and_a) {
@@ -1361,6 +1531,25 @@ int parm_damped_steps(parm_system *s, double dt, double damping, int64_t nsteps)
     return run_steps(s, true, c, nsteps);
 }
 
+int parm_autotune(parm_system *s, double dt, int damped) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(require_ready(s));
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(PARM_E_INVALID, "dt must be finite and > 0");
+    PARM_CHECK(set_device(s));
+    if (!s->built) PARM_CHECK(update_list(s, false, nullptr));
+    // the run's own fused kernel; its outputs go to scratch, so the
+    // coefficients only need to be finite
+    StepCoef c{};
+    c.k0 = dt;
+    c.k1 = dt * dt / 2;
+    c.k2 = dt / 2;
+    c.k3 = 1.0;
+    c.k4 = dt / 2;
+    PARM_CHECK(autotune(s, damped ? kStageDampedKickDrift : kStageVerletKickDrift, c));
+    PARM_HIP(hipStreamSynchronize(s->stream));
+    return PARM_OK;
+}
+
 int parm_profile_enable(parm_system *s, int enable) {
     PARM_CHECK(check_sys(s));
     s->prof = enable != 0;
@@ -1404,6 +1593,446 @@ int parm_synchronize(parm_system *s) {
     PARM_CHECK(check_sys(s));
     PARM_CHECK(set_device(s));
     PARM_HIP(hipStreamSynchronize(s->stream));
+    return PARM_OK;
+}
+
+// ------------------------------------------------------------ snapshots
+int parm_snapshot_begin(parm_system *s, int what, int64_t *ticket) {
+    PARM_CHECK(check_sys(s));
+    if (!ticket) return fail(PARM_E_INVALID, "ticket is null");
+    if (what & ~3) return fail(PARM_E_INVALID, "what: bit 0 positions, bit 1 velocities");
+    PARM_CHECK(set_device(s));
+    const size_t nv = (size_t)s->n * s->ndim;
+    parm_system::Snap &q = s->snap[s->snap_next % parm_system::kSnapRing];
+    if (!s->out_stream &&
+        hipStreamCreateWithFlags(&s->out_stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(PARM_E_DEVICE, "snapshot stream creation failed");
+    if (!q.dev) {
+        PARM_CHECK(dalloc(&q.dev, 2 * std::max<size_t>(nv, 1)));
+        if (hipHostMalloc((void **)&q.host, sizeof(double) * 2 * std::max<size_t>(nv, 1), 0) !=
+            hipSuccess)
+            return fail(PARM_E_NOMEM, "pinned snapshot allocation failed");
+        PARM_HIP(hipEventCreateWithFlags(&q.ready, hipEventDisableTiming));
+        PARM_HIP(hipEventCreateWithFlags(&q.done, hipEventDisableTiming));
+    } else if (q.ticket >= 0) {
+        // the slot's previous copy must have left before it is refilled
+        PARM_HIP(hipEventSynchronize(q.done));
+    }
+    hipStream_t st = s->stream;
+    if (nv > 0) {
+        if (what & 1) PARM_HIP(launch_download(s->ndim, 0, s->arrays(), s->orig, q.dev, st));
+        if (what & 2)
+            PARM_HIP(launch_download(s->ndim, 1, s->arrays(), s->orig, q.dev + nv, st));
+    }
+    PARM_HIP(hipEventRecord(q.ready, st));
+    PARM_HIP(hipStreamWaitEvent(s->out_stream, q.ready, 0));
+    if (nv > 0 && what)
+        PARM_HIP(hipMemcpyAsync(q.host, q.dev, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost,
+                                s->out_stream));
+    PARM_HIP(hipEventRecord(q.done, s->out_stream));
+    for (int k = 0; k < 3; k++) q.L[k] = s->box.L[k];
+    q.what = what;
+    q.ticket = s->snap_next++;
+    *ticket = q.ticket;
+    return PARM_OK;
+}
+
+namespace {
+// The ring slot of a snapshot still held, after its copy arrived.
+int snapshot_slot(parm_system *s, int64_t ticket, parm_system::Snap **out) {
+    if (ticket < 0 || ticket >= s->snap_next ||
+        ticket < s->snap_next - parm_system::kSnapRing)
+        return fail(PARM_E_INVALID, "snapshot " + std::to_string(ticket) +
+                                        " is not held (only the last " +
+                                        std::to_string(parm_system::kSnapRing) + " are)");
+    parm_system::Snap &q = s->snap[ticket % parm_system::kSnapRing];
+    if (q.ticket != ticket) return fail(PARM_E_STATE, "snapshot slot reused");
+    PARM_HIP(hipEventSynchronize(q.done));
+    *out = &q;
+    return PARM_OK;
+}
+}  // namespace
+
+int parm_snapshot_read(parm_system *s, int64_t ticket, double *x, double *v, double *L) {
+    PARM_CHECK(check_sys(s));
+    PARM_CHECK(set_device(s));
+    parm_system::Snap *q = nullptr;
+    PARM_CHECK(snapshot_slot(s, ticket, &q));
+    const size_t nv = (size_t)s->n * s->ndim;
+    if (x) {
+        if (!(q->what & 1)) return fail(PARM_E_STATE, "snapshot holds no positions");
+        std::memcpy(x, q->host, sizeof(double) * nv);
+    }
+    if (v) {
+        if (!(q->what & 2)) return fail(PARM_E_STATE, "snapshot holds no velocities");
+        std::memcpy(v, q->host + nv, sizeof(double) * nv);
+    }
+    if (L)
+        for (int k = 0; k < s->ndim; k++) L[k] = q->L[k];
+    return PARM_OK;
+}
+
+int parm_snapshot_write_xyz(parm_system *s, int64_t ticket, const char *path, int append,
+                            const char *comment, const char *elements, int precision) {
+    PARM_CHECK(check_sys(s));
+    if (!path) return fail(PARM_E_INVALID, "path is null");
+    if (precision < 1 || precision > 40) return fail(PARM_E_INVALID, "precision in [1, 40]");
+    PARM_CHECK(set_device(s));
+    parm_system::Snap *q = nullptr;
+    PARM_CHECK(snapshot_slot(s, ticket, &q));
+    if (!(q->what & 1)) return fail(PARM_E_STATE, "snapshot holds no positions");
+    if (comment && std::strchr(comment, '\n'))
+        return fail(PARM_E_INVALID, "the comment must be one line");
+    // one frame: the atom count, a comment line, then per atom an element
+    // letter and the ndim components of diff(0, x) = remainder(-x, L), as
+    // the reference's drivers write them (src/bin/LJatoms.cpp:130-158,
+    // src/bin/packer.cpp:150-185); %g at `precision` digits is what an
+    // ostream with that precision prints
+    const int d = s->ndim;
+    std::string out;
+    out.reserve((size_t)s->n * (size_t)(d * (precision + 9) + 3) + 64);
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "%d\n", s->n);
+    out += buf;
+    if (comment) out += comment;
+    out += '\n';
+    for (int i = 0; i < s->n; i++) {
+        out += elements ? elements[i] : 'C';
+        for (int k = 0; k < d; k++) {
+            const double r = std::remainder(0.0 - q->host[(size_t)i * d + k], q->L[k]);
+            std::snprintf(buf, sizeof buf, "\t%.*g", precision, r);
+            out += buf;
+        }
+        out += '\n';
+    }
+    FILE *fh = std::fopen(path, append ? "ab" : "wb");
+    if (!fh) return fail(PARM_E_INVALID, std::string("cannot open ") + path);
+    const size_t wrote = std::fwrite(out.data(), 1, out.size(), fh);
+    const int closed = std::fclose(fh);
+    if (wrote != out.size() || closed != 0)
+        return fail(PARM_E_DEVICE, std::string("write to ") + path + " failed");
+    return PARM_OK;
+}
+
+}  // extern "C"
+
+// ================================================ nonlinear CG minimiser
+// CollectionNLCG / CollectionNLCGFixedL (src/collection.hpp:400-474;
+// src/collection.cpp:494-870) on one device system whose collection has the
+// system's pair interaction and NeighborList only.  The scalar logic is the
+// reference's, in its order; every O(N) loop is a device pass.
+struct parm_nlcg {
+    parm_system *s = nullptr;
+    bool fixedL = false;
+    // parameters
+    double dt = 0, seceps = 0;
+    unsigned secmax = 0;
+    double kappa = 0, alphamax = 2.0, afrac = 0, dxmax = 100, stepmax = 1e-3, kmax = 0;
+    double P0 = 0;
+    // kept between iterations
+    double Knew = 0, k = 0, vl = 0, fl = 0, al = 0;
+    // tracking
+    double alpha = 0, beta = 0, betaused = 0, dxsum = 0, alphavmax = 0, maxdV = 0;
+    unsigned sec = 0;
+};
+
+namespace {
+
+double box_V(const parm_system *s) {
+    double V = 1.0;
+    for (int k = 0; k < s->ndim; k++) V *= s->box.L[k];
+    return V;
+}
+
+// get_length_squared (src/collection.cpp:627-635)
+double length_squared(const parm_system *s) {
+    return s->ndim == 3 ? std::pow(box_V(s), 2.0 / 3.0) : box_V(s);
+}
+
+// One vector pass (assignment op, then the sums), read back.
+int nlcg_pass(parm_nlcg *c, int op, double beta, double lfac_all, double out[5]) {
+    parm_system *s = c->s;
+    const double lf[3] = {lfac_all, lfac_all, lfac_all};
+    ReduceBuf R = s->redbuf(kSlotScratch);
+    PARM_HIP(launch_nlcg_vec(s->ndim, op, s->arrays(), beta, lf, R, s->stream));
+    if (!out) return PARM_OK;
+    PARM_CHECK(read_out(s));
+    for (int q = 0; q < 5; q++) out[q] = s->hout[q];
+    return PARM_OK;
+}
+
+// stepx (src/collection.cpp:608-625; FixedL :855-870): x *= Lfac, x += v dx,
+// then the box edges (all, or the first one) times Lfac.
+int nlcg_stepx(parm_nlcg *c, double dx) {
+    parm_system *s = c->s;
+    const int d = s->ndim;
+    const double Lfac = c->fixedL ? std::exp(dx * c->vl / c->kappa)
+                                  : std::exp(dx * c->vl / (c->kappa * d));
+    double lf[3] = {Lfac, Lfac, Lfac};
+    if (c->fixedL) lf[1] = lf[2] = 1.0;
+    PARM_HIP(launch_nlcg_stepx(d, s->arrays(), lf, dx, s->stream));
+    double L[3];
+    for (int k = 0; k < d; k++) L[k] = s->box.L[k];
+    if (c->fixedL)
+        L[0] *= Lfac;
+    else
+        for (int k = 0; k < d; k++) L[k] *= Lfac;
+    return apply_box(s, L);
+}
+
+// Collection::set_forces_get_pressure(false) (src/collection.cpp:181-208):
+// f = pair forces, a = 0 for fixed atoms; returns sum r.f.
+int forces_get_pressure(parm_system *s, double *W) {
+    PARM_CHECK(set_forces(s, false));
+    double zero[3] = {1.0, 1.0, 1.0};
+    ReduceBuf R = s->redbuf(kSlotScratch);
+    PARM_HIP(launch_nlcg_vec(s->ndim, kNlcgFixedA0, s->arrays(), 0.0, zero, R, s->stream));
+    StepCoef cf{};
+    PARM_HIP(launch_force(s->ndim, s->prec, kStageSums, true, s->arrays(), s->listview(),
+                          s->pairview(), s->box, cf, s->redbuf(kSlotScratch), s->stream));
+    PARM_CHECK(read_out(s));
+    *W = s->hout[1];
+    return PARM_OK;
+}
+
+// CollectionNLCG::set_forces(constraints_and_a, setV) (src/collection.cpp:
+// 537-569).
+int nlcg_set_forces(parm_nlcg *c, bool constraints_and_a, bool setV) {
+    parm_system *s = c->s;
+    const double V = box_V(s);
+    if (setV) {
+        double interacP = 0;
+        PARM_CHECK(forces_get_pressure(s, &interacP));
+        c->fl = ((interacP / s->ndim) - (c->P0 * V)) / c->kappa;
+        if (constraints_and_a) {
+            c->al = c->fl;
+            c->vl = c->fl;
+        }
+    } else {
+        PARM_CHECK(set_forces(s, false));
+    }
+    if (constraints_and_a) {
+        double o[5];
+        PARM_CHECK(nlcg_pass(c, kNlcgAssignAV, 0.0, 1.0, o));
+        c->Knew = o[kNlcgFA] / length_squared(s) + c->fl * c->al;  // fdota
+    }
+    return PARM_OK;
+}
+
+// fdotv / vdotv / fdota (src/collection.cpp:650-683) from one pass.
+int nlcg_dots(parm_nlcg *c, double *fdota, double *fdotv, double *vdotv) {
+    double o[5];
+    PARM_CHECK(nlcg_pass(c, kNlcgDots, 0.0, 1.0, o));
+    const double l2 = length_squared(c->s);
+    if (fdota) *fdota = o[kNlcgFA] / l2 + c->fl * c->al;
+    if (fdotv) *fdotv = o[kNlcgFV] / l2 + c->fl * c->vl;
+    if (vdotv) *vdotv = o[kNlcgVV] / l2 + c->vl * c->vl;
+    return PARM_OK;
+}
+
+int nlcg_update_trackers(parm_nlcg *c) { return update_list(c->s, false, nullptr); }
+
+// CollectionNLCG::timestep (src/collection.cpp:685-835).
+int nlcg_timestep(parm_nlcg *c) {
+    PARM_CHECK(nlcg_stepx(c, c->dt));
+    PARM_CHECK(nlcg_set_forces(c, false, true));
+    double fv = 0;
+    PARM_CHECK(nlcg_dots(c, nullptr, &fv, nullptr));
+    double eta0 = -fv;  // slope at x0 - dt
+    double eta;
+    PARM_CHECK(nlcg_stepx(c, -c->dt));
+    PARM_CHECK(nlcg_update_trackers(c));
+    c->alpha = -c->dt;
+    PARM_CHECK(nlcg_set_forces(c, false, true));
+    c->dxsum = 0;
+    double vdv = 0;
+    PARM_CHECK(nlcg_dots(c, nullptr, &fv, &vdv));
+    const int d = c->s->ndim;
+    // secant steps along v until the slope along v is small
+    for (c->sec = 0; c->sec < c->secmax; c->sec++) {
+        eta = -fv;  // the slope at x; fdotv of the last force evaluation
+        double alphafac = -eta / std::fabs(eta0 - eta);
+        if (std::fabs(eta0 - eta) <= 1e-12 * std::fabs(eta)) {
+            alphafac = c->alphamax > 0 ? c->alphamax : 1.1;
+            c->sec = c->sec > 0 ? c->sec * 2 - 1 : 1;
+        }
+        if (c->alphamax > 0 && alphafac > c->alphamax) alphafac = c->alphamax;
+        if (c->alphamax > 0 && alphafac < -c->alphamax) alphafac = -c->alphamax;
+        c->alpha = std::fabs(c->alpha) * alphafac;
+        const double newdxsum = std::fabs(c->dxsum + c->alpha);
+        if (c->dxmax > 0 && newdxsum > c->dxmax) {
+            c->k = 0;
+            break;
+        }
+        const double dVoverV =
+            std::expm1(std::fabs(c->dxsum + c->alpha) * c->vl / (c->kappa * d));
+        if (c->maxdV > 0 && dVoverV > c->maxdV) {
+            c->k = 0;
+            break;
+        }
+        c->dxsum += c->alpha;
+        PARM_CHECK(nlcg_stepx(c, c->alpha));
+        PARM_CHECK(nlcg_set_forces(c, false, true));
+        eta0 = eta;
+        if (c->alpha * c->alpha * vdv < c->seceps * c->seceps) break;
+        if ((c->sec > 1) && (c->afrac > 0) && (std::fabs(c->alpha) < std::fabs(c->dxsum)) &&
+            (std::fabs(c->alpha) / std::fabs(c->dxsum) < c->afrac))
+            break;
+        if (c->stepmax > 0 && c->dxsum * c->dxsum * vdv > c->stepmax * c->stepmax) {
+            c->k = 0;
+            break;
+        }
+        // the next iteration's slope (the reference's fdotv at its top)
+        PARM_CHECK(nlcg_dots(c, nullptr, &fv, nullptr));
+    }
+    c->alphavmax = std::sqrt(c->alpha * c->alpha * vdv);
+    const double Kold = c->Knew;
+    double Kmid = 0;
+    PARM_CHECK(nlcg_dots(c, &Kmid, nullptr, nullptr));
+    // a = f, al = fl, Knew = fdota
+    double o[5];
+    PARM_CHECK(nlcg_pass(c, kNlcgAssignA, 0.0, 1.0, o));
+    c->al = c->fl;
+    c->Knew = o[kNlcgFA] / length_squared(c->s) + c->fl * c->al;
+    c->beta = (c->Knew - Kmid) / Kold;
+    c->betaused = c->beta;
+    c->k++;
+    if (c->k >= c->kmax || std::isinf(c->betaused) || std::isnan(c->betaused) ||
+        c->betaused <= 0) {
+        c->k = 0;
+        c->betaused = 0;
+    } else if (c->betaused > 1) {
+        c->betaused = 1;
+    }
+    PARM_CHECK(nlcg_pass(c, kNlcgAssignV, c->betaused, 1.0, nullptr));
+    c->vl = c->al + c->betaused * c->vl;
+    return PARM_OK;
+}
+
+int check_nlcg(const parm_nlcg *c) {
+    if (!c || !c->s) return fail(PARM_E_INVALID, "null parm_nlcg");
+    return PARM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int parm_nlcg_create(parm_system *s, double dt, double P0, double kappa, double kmax,
+                     int secmax, double seceps, int fixed_L, parm_nlcg **out) {
+    PARM_CHECK(check_sys(s));
+    if (!out) return fail(PARM_E_INVALID, "out is null");
+    *out = nullptr;
+    if (!(kappa > 0)) return fail(PARM_E_INVALID, "kappa must be > 0");
+    if (secmax < 0) return fail(PARM_E_INVALID, "secmax must be >= 0");
+    PARM_CHECK(require_ready(s));
+    PARM_CHECK(set_device(s));
+    // Collection's constructor runs initialize() with Collection::set_forces
+    // (src/collection.cpp:3-19): the NLCG members start at zero
+    PARM_CHECK(parm_initialize(s));
+    parm_nlcg *c = new parm_nlcg();
+    c->s = s;
+    c->fixedL = fixed_L != 0;
+    c->dt = dt;
+    c->P0 = P0;
+    c->kappa = kappa;
+    c->kmax = kmax;
+    c->secmax = (unsigned)secmax;
+    c->seceps = seceps;
+    *out = c;
+    return PARM_OK;
+}
+
+int parm_nlcg_destroy(parm_nlcg *c) {
+    delete c;
+    return PARM_OK;
+}
+
+int parm_nlcg_set_limits(parm_nlcg *c, double alphamax, double afrac, double dxmax,
+                         double stepmax, double maxdV) {
+    PARM_CHECK(check_nlcg(c));
+    c->alphamax = alphamax;
+    c->afrac = afrac;
+    c->dxmax = dxmax;
+    c->stepmax = stepmax;
+    c->maxdV = maxdV;
+    return PARM_OK;
+}
+
+int parm_nlcg_reset(parm_nlcg *c) {
+    PARM_CHECK(check_nlcg(c));
+    PARM_CHECK(set_device(c->s));
+    // reset (src/collection.cpp:528-535): k = 0; set_forces(true, true)
+    // (a = v = f); v = a; vl = al
+    c->k = 0;
+    PARM_CHECK(nlcg_set_forces(c, true, true));
+    c->vl = c->al;
+    return PARM_OK;
+}
+
+int parm_nlcg_set_params(parm_nlcg *c, double dt, double P0, double kappa) {
+    PARM_CHECK(check_nlcg(c));
+    if (!(kappa > 0)) return fail(PARM_E_INVALID, "kappa must be > 0");
+    // set_dt / set_pressure_goal / set_kappa each end in reset()
+    c->dt = dt;
+    c->P0 = P0;
+    c->kappa = kappa;
+    return parm_nlcg_reset(c);
+}
+
+int parm_nlcg_set_forces(parm_nlcg *c, int constraints_and_a, int setV) {
+    PARM_CHECK(check_nlcg(c));
+    PARM_CHECK(set_device(c->s));
+    return nlcg_set_forces(c, constraints_and_a != 0, setV != 0);
+}
+
+int parm_nlcg_timestep(parm_nlcg *c, int64_t nsteps) {
+    PARM_CHECK(check_nlcg(c));
+    if (nsteps < 0) return fail(PARM_E_INVALID, "nsteps < 0");
+    PARM_CHECK(set_device(c->s));
+    for (int64_t q = 0; q < nsteps; q++) PARM_CHECK(nlcg_timestep(c));
+    return PARM_OK;
+}
+
+int parm_nlcg_descend(parm_nlcg *c) {
+    PARM_CHECK(check_nlcg(c));
+    PARM_CHECK(set_device(c->s));
+    // descend (src/collection.cpp:837-853)
+    PARM_CHECK(nlcg_set_forces(c, false, true));
+    PARM_CHECK(nlcg_pass(c, kNlcgDescend, 0.0, 1.0, nullptr));
+    c->al = c->fl;
+    c->vl = c->fl;
+    PARM_CHECK(nlcg_stepx(c, c->dt));
+    return nlcg_update_trackers(c);
+}
+
+int parm_nlcg_observables(parm_nlcg *c, double *H, double *P, double *K) {
+    PARM_CHECK(check_nlcg(c));
+    parm_system *s = c->s;
+    PARM_CHECK(set_device(s));
+    const double V = box_V(s);
+    if (H || P) {
+        double U = 0, W = 0;
+        PARM_CHECK(parm_pair_sums(s, &U, &W, nullptr, nullptr, nullptr));
+        if (H) *H = U + c->P0 * V;                 // hamiltonian (:571-573)
+        if (P) *P = W / V / (double)s->ndim;       // pressure (:597-606)
+    }
+    if (K) {  // kinetic_energy, masses ignored (:575-590)
+        const double Lfac = std::exp(c->vl / (c->kappa * s->ndim));
+        double o[5];
+        PARM_CHECK(nlcg_pass(c, kNlcgDots, 0.0, Lfac, o));
+        *K = o[kNlcgKin] / 2.0;
+    }
+    return PARM_OK;
+}
+
+int parm_nlcg_info(parm_nlcg *c, double *out) {
+    PARM_CHECK(check_nlcg(c));
+    if (!out) return fail(PARM_E_INVALID, "out is null");
+    const double v[12] = {c->alpha, c->beta,     c->betaused, c->dxsum,
+                          c->alphavmax, (double)c->sec, c->k, c->vl,
+                          c->fl,    c->al,       c->Knew,     c->maxdV};
+    for (int q = 0; q < 12; q++) out[q] = v[q];
     return PARM_OK;
 }
 

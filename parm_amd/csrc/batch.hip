@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
 #pragma unroll
         for (int k = 0; k < D; k++) fi[k] = (double)f32[k];
     }
-    double e = acc.e, nc = acc.contacts;
+    double e = acc.e, nc = acc.contacts, w = acc.w, no = acc.overlaps;
 #pragma unroll
     for (int o = 1; o < TPA; o <<= 1) {
 #pragma unroll
@@ -117,12 +117,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
         if (ENERGY) {
             e += __shfl_xor(e, o);
             nc += __shfl_xor(nc, o);
+            w += __shfl_xor(w, o);
+            no += __shfl_xor(no, o);
         }
     }
     if (!act || sub != 0) return;
     if (ENERGY) {
-        eatom[2 * (size_t)i] = 0.5 * e;
-        eatom[2 * (size_t)i + 1] = nc;
+        // the full list visits each pair from both atoms: halves per atom
+        double4 *ea = reinterpret_cast<double4 *>(eatom);
+        ea[i] = double4{0.5 * e, nc, 0.5 * w, no};
     }
     if (STAGE == kStageSums) return;
     // fused kick + drift: f and a are stored by the run's last (plain kick)
@@ -448,30 +451,39 @@ __global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
     }
 }
 
-// Per-system sums of the per-atom energies / contact counts.
+// Per-system sums of the per-atom energies, virials and contact / overlap
+// counts (NListed::energy, pressure, contacts, overlaps per system): one
+// wavefront per system, compensated sums for the energy and the virial.
 __global__ __launch_bounds__(kBlock) void k_batch_sys_sums(int nsys, BatchArrays A,
                                                            const double *__restrict__ eatom,
-                                                           double *__restrict__ U,
-                                                           double *__restrict__ contacts) {
+                                                           double *__restrict__ out) {
     const int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     if (s >= nsys) return;
     const int off = A.geom[s].off, n = A.geom[s].n;
-    KSum e;
-    double c = 0.0;
+    const double4 *ea = reinterpret_cast<const double4 *>(eatom);
+    KSum e, w;
+    double c = 0.0, o = 0.0;
     for (int k = lane; k < n; k += kWave) {
-        e.add(eatom[2 * (size_t)(off + k)]);
-        c += eatom[2 * (size_t)(off + k) + 1];
+        const double4 q = ea[off + k];
+        e.add(q.x);
+        c += q.y;
+        w.add(q.z);
+        o += q.w;
     }
-    CSum cs{e.s, e.c};
+    CSum ce{e.s, e.c}, cw{w.s, w.c};
 #pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        cs = csum_merge(cs, CSum{__shfl_xor(cs.s, o), __shfl_xor(cs.c, o)});
-        c += __shfl_xor(c, o);
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        ce = csum_merge(ce, CSum{__shfl_xor(ce.s, d), __shfl_xor(ce.c, d)});
+        cw = csum_merge(cw, CSum{__shfl_xor(cw.s, d), __shfl_xor(cw.c, d)});
+        c += __shfl_xor(c, d);
+        o += __shfl_xor(o, d);
     }
     if (lane == 0) {
-        U[s] = cs.s + cs.c;
-        contacts[s] = c / 2;
+        out[s] = ce.s + ce.c;
+        out[nsys + s] = cw.s + cw.c;
+        out[2 * (size_t)nsys + s] = c / 2;
+        out[3 * (size_t)nsys + s] = o / 2;
     }
 }
 
@@ -633,10 +645,10 @@ hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const 
 }
 
 hipError_t launch_batch_sys_sums(int nsys, const BatchArrays &A, const double *eatom,
-                                 double *U, double *contacts, hipStream_t s) {
+                                 double *out, hipStream_t s) {
     const int per = kBlock / kWave;
     hipLaunchKernelGGL(k_batch_sys_sums, dim3((nsys + per - 1) / per), dim3(kBlock), 0, s,
-                       nsys, A, eatom, U, contacts);
+                       nsys, A, eatom, out);
     return hipGetLastError();
 }
 

@@ -89,9 +89,13 @@ struct parm_batch {
 
     int pair_kind = kPairNone, ntypes = 0, single = 1, harmonic = 0;
     PairParam *tab = nullptr;
-    double *eatom = nullptr, *Usys = nullptr, *Csys = nullptr;
+    double *eatom = nullptr, *sysout = nullptr;  // 4 per atom; 4 x nsys sums
 
     hipGraphExec_t gexec = nullptr;
+    // instrumentation: while on, runs launch eagerly and every fused step's
+    // force launch and rebuild launch is bracketed by events
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_force, prof_rebuild;
     bool g_damped = false;
     StepCoef g_coef{};
     void *g_pos = nullptr;
@@ -174,8 +178,15 @@ void bdestroy(parm_batch *b) {
     bfree(b->npairs);
     bfree(b->tab);
     bfree(b->eatom);
-    bfree(b->Usys);
-    bfree(b->Csys);
+    bfree(b->sysout);
+    for (auto &p : b->prof_force) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    for (auto &p : b->prof_rebuild) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
     if (b->hgflag) (void)hipHostFree(b->hgflag);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -349,14 +360,36 @@ int bautotune(parm_batch *b) {
 // buffer (with |x_{q+1} - xref|), then update_trackers of step q-1: the
 // rebuild at x_q of the systems whose skin test fired at x_q (recomputing
 // |x_{q+1} - xref| for them), then the skin test at x_{q+1}.
+int prof_begin(parm_batch *b, hipEvent_t *e) {
+    e[0] = e[1] = nullptr;
+    if (!b->prof) return PARM_OK;
+    PB_HIP(hipEventCreate(&e[0]));
+    PB_HIP(hipEventCreate(&e[1]));
+    PB_HIP(hipEventRecord(e[0], b->stream));
+    return PARM_OK;
+}
+
+int prof_end(parm_batch *b, hipEvent_t *e,
+             std::vector<std::pair<hipEvent_t, hipEvent_t>> &dst) {
+    if (!e[0]) return PARM_OK;
+    PB_HIP(hipEventRecord(e[1], b->stream));
+    dst.emplace_back(e[0], e[1]);
+    return PARM_OK;
+}
+
 int fused_step(parm_batch *b, bool damped, const StepCoef &c, void *cur, void *nxt) {
     BatchArrays A = b->arrays();
     A.pos_in = cur;
     A.pos_out = nxt;
+    hipEvent_t ev[2];
+    PB_CHECK(prof_begin(b, ev));
     PB_HIP(launch_batch_force(b->ndim, b->prec,
                               damped ? kStageDampedKickDrift : kStageVerletKickDrift, false, A,
                               b->listview(), b->pairview(), c, b->eatom, b->stream));
+    PB_CHECK(prof_end(b, ev, b->prof_force));
+    PB_CHECK(prof_begin(b, ev));
     PB_CHECK(brebuild(b, false, true, cur, nxt));
+    PB_CHECK(prof_end(b, ev, b->prof_rebuild));
     PB_HIP(launch_batch_skin(b->nsys, b->arrays(), b->skin, b->flags, b->gflag, b->stream));
     return PARM_OK;
 }
@@ -373,7 +406,7 @@ int run_batch(parm_batch *b, bool damped, const StepCoef &c, int64_t nsteps, boo
     PB_HIP(launch_batch_skin(b->nsys, b->arrays(), b->skin, b->flags, b->gflag, st));
     int64_t q = 0;
     const int64_t pairs = (nsteps - 1) / 2;
-    if (graph && pairs > 0) {
+    if (graph && pairs > 0 && !b->prof) {
         const bool same = b->gexec && b->g_damped == damped && b->g_pos == b->pos &&
                           b->g_tpa == b->tpa && b->g_nbr == b->nbr && std::memcmp(&b->g_coef, &c, sizeof c) == 0;
         if (!same) {
@@ -496,8 +529,8 @@ int parm_batch_create(int ndim, int precision, int64_t nsys, const int64_t *nato
         (rc = balloc((char **)&b->xs, n * b->pos_bytes)) || (rc = balloc(&b->dsort, n)) ||
         (rc = balloc(&b->flags, (size_t)nsys)) || (rc = balloc(&b->which, (size_t)nsys)) ||
         (rc = balloc(&b->gflag, 1)) || (rc = balloc(&b->npairs, (size_t)nsys)) ||
-        (rc = balloc(&b->eatom, 2 * n)) || (rc = balloc(&b->Usys, (size_t)nsys)) ||
-        (rc = balloc(&b->Csys, (size_t)nsys)) || (rc = balloc(&b->geom, (size_t)nsys)))
+        (rc = balloc(&b->eatom, 4 * n)) || (rc = balloc(&b->sysout, 4 * (size_t)nsys)) ||
+        (rc = balloc(&b->geom, (size_t)nsys)))
         return bail(rc);
     if (hipHostMalloc((void **)&b->hgflag, sizeof(uint32_t), 0) != hipSuccess)
         return bail(bfail(PARM_E_NOMEM, "pinned allocation failed"));
@@ -572,6 +605,10 @@ static int batch_pair(parm_batch *b, int kind, int ntypes, const double *eps,
     for (int i = 0; i < b->ntot; i++)
         if (atype[i] < 0 || atype[i] >= ntypes)
             return bfail(PARM_E_INVALID, "atom type out of range");
+    const std::string asym = check_rows_symmetric(kind, ntypes, eps, sigma);
+    if (!asym.empty())
+        return bfail(PARM_E_UNSUPPORTED,
+                     asym + ": the device evaluates each pair from both of its atoms");
     std::vector<PairParam> tab;
     bool harm = true;
     make_pair_table(kind, ntypes, eps, sigma, third, tab, harm);
@@ -726,21 +763,71 @@ int parm_batch_verlet_steps(parm_batch *b, double dt, int64_t nsteps, int use_gr
     return run_batch(b, false, c, nsteps, use_graph != 0);
 }
 
-int parm_batch_sums(parm_batch *b, double *U, double *contacts) {
+int parm_batch_pair_sums(parm_batch *b, double *U, double *W, double *contacts,
+                         double *overlaps) {
     if (!b) return bfail(PARM_E_INVALID, "null batch");
     if (b->pair_kind == kPairNone) return bfail(PARM_E_STATE, "no pair interaction set");
     PB_HIP(hipSetDevice(b->device));
     StepCoef c{};
     PB_HIP(launch_batch_force(b->ndim, b->prec, kStageSums, true, b->arrays(), b->listview(),
                               b->pairview(), c, b->eatom, b->stream));
-    PB_HIP(launch_batch_sys_sums(b->nsys, b->arrays(), b->eatom, b->Usys, b->Csys, b->stream));
-    if (U)
-        PB_HIP(hipMemcpyAsync(U, b->Usys, sizeof(double) * b->nsys, hipMemcpyDeviceToHost,
-                              b->stream));
-    if (contacts)
-        PB_HIP(hipMemcpyAsync(contacts, b->Csys, sizeof(double) * b->nsys,
-                              hipMemcpyDeviceToHost, b->stream));
+    PB_HIP(launch_batch_sys_sums(b->nsys, b->arrays(), b->eatom, b->sysout, b->stream));
+    double *dst[4] = {U, W, contacts, overlaps};
+    for (int q = 0; q < 4; q++)
+        if (dst[q])
+            PB_HIP(hipMemcpyAsync(dst[q], b->sysout + (size_t)q * b->nsys,
+                                  sizeof(double) * b->nsys, hipMemcpyDeviceToHost, b->stream));
     PB_HIP(hipStreamSynchronize(b->stream));
+    return PARM_OK;
+}
+
+int parm_batch_sums(parm_batch *b, double *U, double *contacts) {
+    return parm_batch_pair_sums(b, U, nullptr, contacts, nullptr);
+}
+
+int parm_batch_autotune(parm_batch *b) {
+    if (!b) return bfail(PARM_E_INVALID, "null batch");
+    if (b->pair_kind == kPairNone) return bfail(PARM_E_STATE, "no pair interaction set");
+    if (!b->built) return bfail(PARM_E_STATE, "call parm_batch_initialize first");
+    PB_HIP(hipSetDevice(b->device));
+    PB_CHECK(bautotune(b));
+    PB_HIP(hipStreamSynchronize(b->stream));
+    return PARM_OK;
+}
+
+int parm_batch_profile_enable(parm_batch *b, int enable) {
+    if (!b) return bfail(PARM_E_INVALID, "null batch");
+    b->prof = enable != 0;
+    return PARM_OK;
+}
+
+int parm_batch_profile_read(parm_batch *b, int reset, double *force_ms,
+                            int64_t *force_launches, double *rebuild_ms,
+                            int64_t *rebuild_launches) {
+    if (!b) return bfail(PARM_E_INVALID, "null batch");
+    PB_HIP(hipSetDevice(b->device));
+    PB_HIP(hipStreamSynchronize(b->stream));
+    auto total = [](const std::vector<std::pair<hipEvent_t, hipEvent_t>> &v, double *ms) {
+        double t = 0.0;
+        for (const auto &p : v) {
+            float e = 0.0f;
+            if (hipEventElapsedTime(&e, p.first, p.second) == hipSuccess) t += e;
+        }
+        if (ms) *ms = t;
+    };
+    total(b->prof_force, force_ms);
+    total(b->prof_rebuild, rebuild_ms);
+    if (force_launches) *force_launches = (int64_t)b->prof_force.size();
+    if (rebuild_launches) *rebuild_launches = (int64_t)b->prof_rebuild.size();
+    if (reset) {
+        for (auto *v : {&b->prof_force, &b->prof_rebuild}) {
+            for (auto &p : *v) {
+                (void)hipEventDestroy(p.first);
+                (void)hipEventDestroy(p.second);
+            }
+            v->clear();
+        }
+    }
     return PARM_OK;
 }
 

@@ -55,6 +55,17 @@ struct BuildReport {
     uint32_t flag;
 };
 
+// NeighborList::ignore exclusions (src/trackers.hpp:190-193): excluded
+// partners of each atom as a CSR over original atom indices (both directions
+// stored, partners sorted); orig maps a device slot to its original index.
+// any == 0 disables the test.
+struct ExclView {
+    const int32_t *orig;
+    const int32_t *off;   // natoms + 1
+    const int32_t *part;  // off[natoms] partners
+    int any;
+};
+
 // Builds the full list (and xref = pos); stats[0] = sum of counts, stats[1] =
 // max count; part holds 2 * build_blocks(n) scratch words.  report: a
 // device-visible pinned BuildReport, or null.
@@ -66,7 +77,8 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                              int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                              int32_t *ncount, uint16_t *lcount, void *xref,
                              unsigned long long *part, unsigned long long *stats,
-                             uint32_t *flag, BuildReport *report, hipStream_t s);
+                             uint32_t *flag, const ExclView &ex, BuildReport *report,
+                             hipStream_t s);
 
 // Tiled layout (tiles.hpp).  launch_tiles fills tiles (cap_tiles records)
 // and info[0] = tiles, info[1] = largest staged stencil, info[2] != 0 when the
@@ -88,7 +100,8 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const int4 
                               int npad, bool out32, void *nbr, int32_t *ncount,
                               uint16_t *lcount, void *xref, unsigned long long *part,
                               unsigned long long *stats, uint32_t *flag, const int *spec,
-                              int uni, BuildReport *report, hipStream_t s);
+                              int uni, const ExclView &ex, BuildReport *report,
+                              hipStream_t s);
 hipError_t launch_tiled_to_global(int ndim, int ntiles, const int4 *tiles,
                                   const uint32_t *keys, const int32_t *cstart,
                                   const CellGrid &g, const uint16_t *nb16,
@@ -258,7 +271,7 @@ hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const 
                                 uint32_t *which, unsigned long long *npairs, uint32_t *gflag,
                                 hipStream_t s);
 hipError_t launch_batch_sys_sums(int nsys, const BatchArrays &A, const double *eatom,
-                                 double *U, double *contacts, hipStream_t s);
+                                 double *out, hipStream_t s);
 constexpr int kBatchMaxCellsPerSystem = 4096;
 
 // Sets the calling thread's parm_last_error() message; returns code.

@@ -243,6 +243,22 @@ __global__ __launch_bounds__(kBlock) void k_cell_bounds(
 //   in list  <=>  sqrt(d2) < (d_i + d_j)/2 + skin   (correctly rounded sqrt).
 // sqrt is only evaluated when d2 is within 1e-14 (relative) of c^2; outside
 // that band the comparison with c^2 decides it with margin to spare.
+// Is the pair (original atoms oi, oj) excluded (NeighborList::ignore)?
+// Binary search of oi's sorted partner list.
+__device__ __forceinline__ bool excluded(const ExclView &ex, int oi, int oj) {
+    int lo = ex.off[oi], hi = ex.off[oi + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = ex.part[mid];
+        if (v == oj) return true;
+        if (v < oj)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return false;
+}
+
 template <int D, int G, int U>
 __global__ __launch_bounds__(kBlock) void k_build_list(
     int n, const typename PosStore<D>::T *__restrict__ pos,
@@ -254,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
     int32_t *__restrict__ ncount, uint16_t *__restrict__ lcount,
     typename PosStore<D>::T *__restrict__ xref,
     unsigned long long *__restrict__ part /* per workgroup: sum, max */,
-    uint32_t *__restrict__ flag) {
+    uint32_t *__restrict__ flag, ExclView ex) {
     __shared__ unsigned long long red[kBlock / kWave][2];
     const int gt = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const int i = gt / G;
@@ -275,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
             const Vec<D> xi = PosStore<D>::load(pi);
             const Vec<D> wi = PosStore<D>::load(xw[i]);
             const double di = diam[i];
+            const int oi = ex.any ? ex.orig[i] : 0;
             uint32_t key = keys[i];
             int ci[3] = {0, 0, 0};
 #pragma unroll
@@ -388,6 +405,9 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
                                 else if (d2w <= c2 * (1.0 + 1e-6))
                                     pass = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
                                                       dj[u], box, skin);
+                                // ignored pairs never enter the list
+                                // (src/trackers.cpp:64)
+                                if (ex.any && pass) pass = !excluded(ex, oi, ex.orig[j]);
                             }
                             const unsigned long long m =
                                 (__ballot(pass) >> gbase) &
@@ -637,7 +657,7 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     double skin, double dmax, int tpa, int cap, int npad, void *__restrict__ nbr,
     int32_t *__restrict__ ncount, uint16_t *__restrict__ lcount,
     typename PosStore<D>::T *__restrict__ xref, unsigned long long *__restrict__ part,
-    uint32_t *__restrict__ flag, const int *__restrict__ spec) {
+    uint32_t *__restrict__ flag, const int *__restrict__ spec, ExclView ex) {
     constexpr int NT = kWave * G;
     using Slot = BuildSlot<D, UNI>;
     extern __shared__ float st_raw[];
@@ -728,6 +748,7 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
             const Vec<D> wi = PosStore<D>::load(xw[i]);
             const double di = diam[i];
             const float dif = (float)di;
+            const int oi = ex.any ? ex.orig[i] : 0;
             const int cx = (int)(keys[i] % (uint32_t)g.nc[0]);
             // along x the minimum image is explicit for a collapsed
             // dimension or a full-row stencil (no staged shift)
@@ -814,6 +835,18 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                                     const int j = L.g0[cj] + (s - L.off[cj]);
                                     pass[u] = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
                                                          diam[j], box, skin);
+                                }
+                            }
+                        }
+                        // ignored pairs never enter the list (src/trackers.cpp:64)
+                        if (ex.any) {
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                if (pass[u]) {
+                                    const int s = base + u * G + sub;
+                                    const int cj = slot_cell(L, s);
+                                    const int j = L.g0[cj] + (s - L.off[cj]);
+                                    pass[u] = !excluded(ex, oi, ex.orig[j]);
                                 }
                             }
                         }
@@ -1065,7 +1098,8 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
                              int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                              int32_t *ncount, uint16_t *lcount, void *xref,
                              unsigned long long *part, unsigned long long *stats,
-                             uint32_t *flag, BuildReport *report, hipStream_t s) {
+                             uint32_t *flag, const ExclView &ex, BuildReport *report,
+                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int nb = build_blocks(n);
     if (ndim == 3) {
@@ -1074,14 +1108,14 @@ hipError_t launch_build_list(int ndim, int n, const void *pos, void *xw,
         hipLaunchKernelGGL((k_build_list<3, kBuildLanes, kBuildUnroll>), dim3(nb), dim3(kBlock), 0, s,
                            n, (const double4 *)pos, (const double4 *)xw, rank, diam, type,
                            keys, cstart, g, box, skin, dmax, type_bits, tpa, cap, npad,
-                           nbr, ncount, lcount, (double4 *)xref, part, flag);
+                           nbr, ncount, lcount, (double4 *)xref, part, flag, ex);
     } else {
         hipLaunchKernelGGL(k_wrap<2>, dim3(nblk(n)), dim3(kBlock), 0, s, n,
                            (const double2 *)pos, g, box, (double2 *)xw);
         hipLaunchKernelGGL((k_build_list<2, kBuildLanes, kBuildUnroll>), dim3(nb), dim3(kBlock), 0, s,
                            n, (const double2 *)pos, (const double2 *)xw, rank, diam, type,
                            keys, cstart, g, box, skin, dmax, type_bits, tpa, cap, npad,
-                           nbr, ncount, lcount, (double2 *)xref, part, flag);
+                           nbr, ncount, lcount, (double2 *)xref, part, flag, ex);
     }
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kStatsThreads), 0, s, nb, part, stats,
                        flag, (const int *)nullptr, report);
@@ -1123,7 +1157,8 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
                               int npad, bool out32, void *nbr, int32_t *ncount,
                               uint16_t *lcount, void *xref, unsigned long long *part,
                               unsigned long long *stats, uint32_t *flag, const int *spec,
-                              int uni, BuildReport *report, hipStream_t s) {
+                              int uni, const ExclView &ex, BuildReport *report,
+                              hipStream_t s) {
     if (n <= 0 || ntiles <= 0) return hipSuccess;
     const size_t slot = (size_t)(uni ? ndim : ndim + 1) * sizeof(float);
     const size_t lds = (size_t)(smax + kBuildPad) * slot +
@@ -1134,7 +1169,7 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
                        dim3(kWave * kTiledLanes), lds, s, tiles, truns, smax, (const PT *)pos, \
                        (const PT *)xw, rank, diam, type, type_bits, keys, cstart, g, box,     \
                        skin, dmax, tpa, cap, npad, nbr, ncount, lcount, (PT *)xref, part,     \
-                       flag, spec)
+                       flag, spec, ex)
 #define PARM_BT_S(DD, PT, O32, UU)                       \
     if (sparse) {                                       \
         PARM_BT(DD, PT, O32, UU, kTiledUnrollSparse);   \

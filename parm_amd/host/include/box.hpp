@@ -108,7 +108,10 @@ class DeviceState {
     double skin = 0;
     std::vector<int32_t> members;
     std::vector<double> diameters;
-    int pair_kind = PARM_PAIR_NONE;
+    // NeighborList::ignore pairs, pushed after the list configuration
+    std::vector<int32_t> ign_a, ign_b;
+    size_t ign_pushed = 0;
+    int pair_form = PARM_PAIR_NONE;  // enum parm_pair_kind
     int ntypes = 0;
     std::vector<double> t_eps, t_sigma, t_third;
     std::vector<int32_t> atom_type;
@@ -144,16 +147,39 @@ class DeviceState {
             parm_check(parm_nlist_set(sys, skin, (int64_t)members.size(), members.data(),
                                       diameters.data()));
             list_dirty = false;
+            ign_pushed = 0;  // a new device list starts without exclusions
+        }
+        if (ign_pushed < ign_a.size()) {
+            parm_check(parm_nlist_ignore(sys, (int64_t)(ign_a.size() - ign_pushed),
+                                         ign_a.data() + ign_pushed, ign_b.data() + ign_pushed));
+            ign_pushed = ign_a.size();
         }
         if (pair_dirty && ntypes > 0) {
             atom_type.resize(n, 0);
-            if (pair_kind == PARM_PAIR_LJ_ATTRACT_REPULSE)
-                parm_check(parm_pair_lj_attract_repulse(sys, ntypes, t_eps.data(),
-                                                        t_sigma.data(), t_third.data(),
-                                                        atom_type.data()));
-            else
-                parm_check(parm_pair_repulsion(sys, ntypes, t_eps.data(), t_sigma.data(),
-                                               t_third.data(), atom_type.data()));
+            const double *e = t_eps.data(), *sg = t_sigma.data(), *th = t_third.data();
+            const int32_t *ty = atom_type.data();
+            switch (pair_form) {
+                case PARM_PAIR_LJ_ATTRACT_REPULSE:
+                    parm_check(parm_pair_lj_attract_repulse(sys, ntypes, e, sg, th, ty));
+                    break;
+                case PARM_PAIR_REPULSION:
+                    parm_check(parm_pair_repulsion(sys, ntypes, e, sg, th, ty));
+                    break;
+                case PARM_PAIR_LJ_CUT:
+                    parm_check(parm_pair_lj_cut(sys, ntypes, e, sg, th, ty));
+                    break;
+                case PARM_PAIR_LJ_CUT_INDEXED:
+                    parm_check(parm_pair_lj_cut_indexed(sys, ntypes, e, sg, th, ty));
+                    break;
+                case PARM_PAIR_LJ_REPULSE:
+                    parm_check(parm_pair_lj_repulse(sys, ntypes, e, sg, ty));
+                    break;
+                case PARM_PAIR_REPULSION_INDEXED:
+                    parm_check(parm_pair_repulsion_indexed(sys, ntypes, e, sg, th, ty));
+                    break;
+                default:
+                    throw std::runtime_error("unknown pair form");
+            }
             pair_dirty = false;
         }
         if (host_dirty) {

@@ -1,16 +1,22 @@
-// interaction.hpp -- drop-in for ParM's Interaction, IEpsSigCutAtom /
-// LJAttractRepulsePair, EpsSigExpAtom / RepulsionPair and NListed<A,P>
-// (src/interaction.hpp:59-104, 989-1018, 1251-1299, 1454-1466, 1528-1590,
-// 1876-1945, 2101-2291) on the MI355X device path.
+// interaction.hpp -- drop-in for ParM's Interaction, NListed<A,P> and the
+// neighbour-listed pair types the device path runs (src/interaction.hpp:
+// 59-104, 119-147, 238-281, 857-987, 989-1018, 1251-1299, 1454-1590,
+// 1876-1945, 2101-2291):
+//   NListed<IEpsSigCutAtom, LJAttractRepulsePair>  (SWIG LJAttractRepulse)
+//   NListed<EpsSigExpAtom, RepulsionPair>          (SWIG Repulsion)
+//   NListed<EpsSigCutAtom, LennardJonesCutPair>    (src/bin/LJatoms.cpp)
+//   NListed<IEpsISigCutAtom, LennardJonesCutPair>  (SWIG LJIICut)
+//   NListed<EpsSigAtom, LJRepulsePair>             (SWIG LJRepulse)
+//   NListed<IEpsISigExpAtom, RepulsionPair>        (SWIG RepulsionII)
 //
 // NListed<A,P> keeps the reference's interface.  Its per-atom parameters are
 // folded into a type table (one type per distinct parameter tuple) that
 // configures the device; energy, forces, pressure, stress, contacts and
-// overlaps then run on the device over the device list.  Only the two pair
-// types of the device path are provided.
+// overlaps then run on the device over the device list.
 #ifndef INTERACTION_H
 #define INTERACTION_H
 
+#include <algorithm>
 #include <map>
 #include <tuple>
 #include <typeinfo>
@@ -71,6 +77,69 @@ struct EpsSigExpAtom : public AtomID {
     flt max_size() { return sigma; }
 };
 
+struct EpsSigAtom : public AtomID {
+    flt epsilon, sigma;
+    EpsSigAtom() {}
+    EpsSigAtom(AtomID a, flt epsilon, flt sigma) : AtomID(a), epsilon(epsilon), sigma(sigma) {}
+    EpsSigAtom(AtomID a, EpsSigAtom other)
+        : AtomID(a), epsilon(other.epsilon), sigma(other.sigma) {}
+    flt max_size() { return sigma; }
+};
+
+struct EpsSigCutAtom : public EpsSigAtom {
+    flt sigcut;
+    EpsSigCutAtom() {}
+    EpsSigCutAtom(AtomID a, flt epsilon, flt sigma, flt cut)
+        : EpsSigAtom(a, epsilon, sigma), sigcut(cut) {}
+    EpsSigCutAtom(AtomID a, EpsSigCutAtom other) : EpsSigAtom(a, other), sigcut(other.sigcut) {}
+    flt max_size() { return sigma * sigcut; }
+};
+
+//! Row-indexed epsilons and sigmas: the pair (a1, a2) takes
+//! a1.epsilons[a2.indx] and a1.sigmas[a2.indx].
+struct IEpsISigCutAtom : public AtomID {
+    vector<flt> epsilons, sigmas;
+    uint indx;
+    flt sigcut;
+    IEpsISigCutAtom() {}
+    IEpsISigCutAtom(AtomID a, vector<flt> epsilons, vector<flt> sigmas, uint indx, flt cut)
+        : AtomID(a), epsilons(epsilons), sigmas(sigmas), indx(indx), sigcut(cut) {
+        if (sigmas.size() != epsilons.size())
+            throw std::invalid_argument("IEpsISigCutAtom: epsilons and sigmas differ in size");
+    }
+    IEpsISigCutAtom(AtomID a, IEpsISigCutAtom other)
+        : AtomID(a), epsilons(other.epsilons), sigmas(other.sigmas), indx(other.indx),
+          sigcut(other.sigcut) {}
+    flt get_epsilon(IEpsISigCutAtom &other) { return row_at(epsilons, other.indx); }
+    flt get_sigma(IEpsISigCutAtom &other) { return row_at(sigmas, other.indx); }
+    flt max_size() { return *std::max_element(sigmas.begin(), sigmas.end()) * sigcut; }
+    static flt row_at(const vector<flt> &row, uint i) {
+        if (i >= row.size()) throw std::invalid_argument("parameter index out of range");
+        return row[i];
+    }
+};
+
+struct IEpsISigExpAtom : public AtomID {
+    vector<flt> epsilons, sigmas;
+    flt exponent;
+    uint indx;
+    IEpsISigExpAtom() {}
+    IEpsISigExpAtom(AtomID a, vector<flt> epsilons, vector<flt> sigmas, uint indx,
+                    flt exponent = 2.5)
+        : AtomID(a), epsilons(epsilons), sigmas(sigmas), exponent(exponent), indx(indx) {
+        if (sigmas.size() != epsilons.size())
+            throw std::invalid_argument("IEpsISigExpAtom: epsilons and sigmas differ in size");
+    }
+    IEpsISigExpAtom(AtomID a, IEpsISigExpAtom other)
+        : AtomID(a), epsilons(other.epsilons), sigmas(other.sigmas),
+          exponent(other.exponent), indx(other.indx) {}
+    flt get_epsilon(IEpsISigExpAtom &other) {
+        return IEpsISigCutAtom::row_at(epsilons, other.indx);
+    }
+    flt get_sigma(IEpsISigExpAtom &other) { return IEpsISigCutAtom::row_at(sigmas, other.indx); }
+    flt max_size() { return *std::max_element(sigmas.begin(), sigmas.end()); }
+};
+
 // ------------------------------------------------------------- pairs
 //! Single-pair evaluation (the reference's pair structs); the bulk sums of
 //! NListed run on the device, these serve NListed::energy(box, pair) and
@@ -122,6 +191,12 @@ struct RepulsionPair {
           exponent((a1.exponent + a2.exponent) / 2.0),
           atom1(a1),
           atom2(a2) {}
+    RepulsionPair(IEpsISigExpAtom a1, IEpsISigExpAtom a2)
+        : eps(a1.get_epsilon(a2)),
+          sig(a1.get_sigma(a2)),
+          exponent((a1.exponent + a2.exponent) / 2.0),
+          atom1(a1),
+          atom2(a2) {}
     flt energy(Box &box) {
         Vec rij = box.diff(atom1.peek().x, atom2.peek().x);
         flt dsq = rij.squaredNorm();
@@ -138,13 +213,102 @@ struct RepulsionPair {
     }
 };
 
+//! LennardJonesCut (src/interaction.hpp:238-281): eps((1 - r^-6)^2 - 1),
+//! shifted to zero at the cut, the force not shifted.
+struct LennardJonesCutPair {
+    flt epsilon, sigma, cut_distance, cut_energy;
+    AtomID atom1, atom2;
+    LennardJonesCutPair(EpsSigCutAtom a1, EpsSigCutAtom a2)
+        : epsilon(std::sqrt(a1.epsilon * a2.epsilon)),
+          sigma((a1.sigma + a2.sigma) / 2),
+          cut_distance(std::max(a1.sigcut, a2.sigcut)),
+          atom1(a1),
+          atom2(a2) {
+        set_cut();
+    }
+    LennardJonesCutPair(IEpsISigCutAtom a1, IEpsISigCutAtom a2)
+        : epsilon(a1.get_epsilon(a2)),
+          sigma(a1.get_sigma(a2)),
+          cut_distance(std::max(a1.sigcut, a2.sigcut)),
+          atom1(a1),
+          atom2(a2) {
+        set_cut();
+    }
+    void set_cut() {
+        flt rsix = std::pow(cut_distance, 6);
+        flt mid = (1 - 1 / rsix);
+        cut_energy = epsilon * (mid * mid - 1);
+    }
+    flt energy(Box &box) {
+        Vec rij = box.diff(atom1.peek().x, atom2.peek().x);
+        flt rsq = rij.squaredNorm() / (sigma * sigma);
+        if (rsq > (cut_distance * cut_distance)) return 0;
+        flt rsix = rsq * rsq * rsq;
+        flt mid = (1 - 1 / rsix);
+        return epsilon * (mid * mid - 1) - cut_energy;
+    }
+    Vec forces(Box &box) {
+        Vec rij = box.diff(atom1.peek().x, atom2.peek().x);
+        flt dsq = rij.squaredNorm();
+        flt rsq = dsq / (sigma * sigma);
+        if (rsq > (cut_distance * cut_distance)) return Vec::Zero();
+        flt rsix = rsq * rsq * rsq;
+        flt fmagTimesR = 12 * epsilon / rsix * (1 / rsix - 1);
+        return rij * (fmagTimesR / dsq);
+    }
+};
+
+//! Repulsive LJ cut at sigma (LJRepulsive, src/interaction.hpp:119-147).
+struct LJRepulsePair {
+    flt epsilon, sigma;
+    AtomID atom1, atom2;
+    LJRepulsePair(EpsSigAtom a1, EpsSigAtom a2)
+        : epsilon(std::sqrt(a1.epsilon * a2.epsilon)),
+          sigma((a1.sigma + a2.sigma) / 2),
+          atom1(a1),
+          atom2(a2) {}
+    flt energy(Box &box) {
+        Vec rij = box.diff(atom1.peek().x, atom2.peek().x);
+        flt rsq = rij.squaredNorm() / (sigma * sigma);
+        if (rsq > 1) return 0;
+        flt rsix = rsq * rsq * rsq;
+        flt mid = (1 - 1 / rsix);
+        return epsilon * (mid * mid);
+    }
+    Vec forces(Box &box) {
+        Vec rij = box.diff(atom1.peek().x, atom2.peek().x);
+        flt dsq = rij.squaredNorm();
+        flt rsq = dsq / (sigma * sigma);
+        if (rsq > 1) return Vec::Zero();
+        flt rsix = rsq * rsq * rsq;
+        flt fmagTimesR = 12 * epsilon / rsix * (1 / rsix - 1);
+        return rij * (fmagTimesR / dsq);
+    }
+};
+
 // --------------------------------------------- device parameter tables
-template <class A>
-struct DeviceAtomTraits;
+//! How an NListed<A, P> folds its atoms into the device's type table: the
+//! C ABI pair form and, per distinct parameter tuple (a "type"), the
+//! parameter rows it passes.
+template <class A, class P>
+struct DevicePairTraits;
+
+//! Row-indexed parameters of types t1, t2 (the rows of t1 at the index of t2).
+template <class K>
+static void fill_rows(const vector<K> &types, vector<double> &out, int row_field) {
+    const size_t nt = types.size();
+    out.assign(nt * nt, 0.0);
+    for (size_t t1 = 0; t1 < nt; t1++)
+        for (size_t t2 = 0; t2 < nt; t2++) {
+            const vector<flt> &row =
+                row_field == 0 ? std::get<0>(types[t1]) : std::get<1>(types[t1]);
+            out[t1 * nt + t2] = IEpsISigCutAtom::row_at(row, std::get<2>(types[t2]));
+        }
+}
 
 template <>
-struct DeviceAtomTraits<IEpsSigCutAtom> {
-    static const int kind = PARM_PAIR_LJ_ATTRACT_REPULSE;
+struct DevicePairTraits<IEpsSigCutAtom, LJAttractRepulsePair> {
+    static const int form = PARM_PAIR_LJ_ATTRACT_REPULSE;
     typedef std::tuple<vector<flt>, uint, flt, flt> Key;
     static Key key(const IEpsSigCutAtom &a) {
         return Key(a.epsilons, a.indx, a.sigma, a.sigcut);
@@ -169,8 +333,8 @@ struct DeviceAtomTraits<IEpsSigCutAtom> {
 };
 
 template <>
-struct DeviceAtomTraits<EpsSigExpAtom> {
-    static const int kind = PARM_PAIR_REPULSION;
+struct DevicePairTraits<EpsSigExpAtom, RepulsionPair> {
+    static const int form = PARM_PAIR_REPULSION;
     typedef std::tuple<flt, flt, flt> Key;
     static Key key(const EpsSigExpAtom &a) { return Key(a.eps, a.sigma, a.exponent); }
     static void table(const vector<Key> &types, DeviceState &d) {
@@ -183,6 +347,71 @@ struct DeviceAtomTraits<EpsSigExpAtom> {
             d.t_sigma[t] = std::get<1>(types[t]);
             d.t_third[t] = std::get<2>(types[t]);
         }
+    }
+};
+
+template <>
+struct DevicePairTraits<EpsSigCutAtom, LennardJonesCutPair> {
+    static const int form = PARM_PAIR_LJ_CUT;
+    typedef std::tuple<flt, flt, flt> Key;
+    static Key key(const EpsSigCutAtom &a) { return Key(a.epsilon, a.sigma, a.sigcut); }
+    static void table(const vector<Key> &types, DeviceState &d) {
+        const size_t nt = types.size();
+        d.t_eps.resize(nt);
+        d.t_sigma.resize(nt);
+        d.t_third.resize(nt);
+        for (size_t t = 0; t < nt; t++) {
+            d.t_eps[t] = std::get<0>(types[t]);
+            d.t_sigma[t] = std::get<1>(types[t]);
+            d.t_third[t] = std::get<2>(types[t]);
+        }
+    }
+};
+
+template <>
+struct DevicePairTraits<IEpsISigCutAtom, LennardJonesCutPair> {
+    static const int form = PARM_PAIR_LJ_CUT_INDEXED;
+    typedef std::tuple<vector<flt>, vector<flt>, uint, flt> Key;
+    static Key key(const IEpsISigCutAtom &a) {
+        return Key(a.epsilons, a.sigmas, a.indx, a.sigcut);
+    }
+    static void table(const vector<Key> &types, DeviceState &d) {
+        fill_rows(types, d.t_eps, 0);
+        fill_rows(types, d.t_sigma, 1);
+        d.t_third.resize(types.size());
+        for (size_t t = 0; t < types.size(); t++) d.t_third[t] = std::get<3>(types[t]);
+    }
+};
+
+template <>
+struct DevicePairTraits<EpsSigAtom, LJRepulsePair> {
+    static const int form = PARM_PAIR_LJ_REPULSE;
+    typedef std::tuple<flt, flt> Key;
+    static Key key(const EpsSigAtom &a) { return Key(a.epsilon, a.sigma); }
+    static void table(const vector<Key> &types, DeviceState &d) {
+        const size_t nt = types.size();
+        d.t_eps.resize(nt);
+        d.t_sigma.resize(nt);
+        d.t_third.assign(nt, 0.0);
+        for (size_t t = 0; t < nt; t++) {
+            d.t_eps[t] = std::get<0>(types[t]);
+            d.t_sigma[t] = std::get<1>(types[t]);
+        }
+    }
+};
+
+template <>
+struct DevicePairTraits<IEpsISigExpAtom, RepulsionPair> {
+    static const int form = PARM_PAIR_REPULSION_INDEXED;
+    typedef std::tuple<vector<flt>, vector<flt>, uint, flt> Key;
+    static Key key(const IEpsISigExpAtom &a) {
+        return Key(a.epsilons, a.sigmas, a.indx, a.exponent);
+    }
+    static void table(const vector<Key> &types, DeviceState &d) {
+        fill_rows(types, d.t_eps, 0);
+        fill_rows(types, d.t_sigma, 1);
+        d.t_third.resize(types.size());
+        for (size_t t = 0; t < types.size(); t++) d.t_third[t] = std::get<3>(types[t]);
     }
 };
 
@@ -208,7 +437,7 @@ class NListed : public Interaction {
     void push_params() {
         claim();
         if (!params_dirty) return;
-        typedef DeviceAtomTraits<A> T;
+        typedef DevicePairTraits<A, P> T;
         std::map<typename T::Key, int32_t> ids;
         vector<typename T::Key> types;
         DeviceState &d = dev();
@@ -233,7 +462,7 @@ class NListed : public Interaction {
                                      "parameter sets per interaction");
         T::table(types, d);
         d.ntypes = (int)types.size();
-        d.pair_kind = T::kind;
+        d.pair_form = T::form;
         d.pair_dirty = true;
         params_dirty = false;
     }

@@ -9,7 +9,9 @@
 #ifndef TRACKERS_H
 #define TRACKERS_H
 
+#include <algorithm>
 #include <cassert>
+#include <set>
 
 #include "box.hpp"
 
@@ -32,6 +34,7 @@ class NeighborList : public StateTracker {
     vector<flt> diameters;
     vector<IDPair> curpairs;  // export cache
     uint cached_which = UINT_MAX;
+    std::set<std::pair<uint, uint> > ignored;  // (larger, smaller) atom index
 
     DeviceState &dev() const { return vecp->device(); }
     void claim() {
@@ -107,9 +110,14 @@ class NeighborList : public StateTracker {
         parm_check(parm_nlist_numpairs(dev().sys, &n));
         return (uint)n;
     }
-    //! Exclusions are not on the device path (it fails loudly instead).
-    void ignore(AtomID, AtomID) {
-        throw std::runtime_error("NeighborList::ignore is not supported by the device path");
+    //! src/trackers.hpp:190-193: the pair never enters the list; the next
+    //! update rebuilds.  Stored once per unordered pair (PairList).
+    void ignore(AtomID a, AtomID b) {
+        uint hi = std::max(a.n(), b.n()), lo = std::min(a.n(), b.n());
+        ignored.insert(std::make_pair(hi, lo));
+        DeviceState &d = dev();
+        d.ign_a.push_back((int32_t)a.n());
+        d.ign_b.push_back((int32_t)b.n());
     }
     void add(AtomID a, flt diameter) {
         atoms.add(a);
@@ -119,7 +127,7 @@ class NeighborList : public StateTracker {
         d.diameters.push_back(diameter);
         d.list_dirty = true;  // like ignorechanged: rebuild at the next update
     }
-    uint ignore_size() const { return 0; }
+    uint ignore_size() const { return (uint)ignored.size(); }
     uint size() const { return atoms.size(); }
     vector<IDPair>::iterator begin() {
         refresh_pairs();

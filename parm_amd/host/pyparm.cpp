@@ -65,6 +65,9 @@ struct type_caster<Matrix> {
 
 typedef NListed<EpsSigExpAtom, RepulsionPair> Repulsion;
 typedef NListed<IEpsSigCutAtom, LJAttractRepulsePair> LJAttractRepulse;
+typedef NListed<EpsSigAtom, LJRepulsePair> LJRepulse;
+typedef NListed<IEpsISigCutAtom, LennardJonesCutPair> LJIICut;
+typedef NListed<IEpsISigExpAtom, RepulsionPair> RepulsionII;
 
 // Bulk state access (extension): natoms x NDIM arrays in AtomVec order.
 static py::array_t<double> get_field(AtomVec &av, int which) {
@@ -274,8 +277,37 @@ PYBIND11_MODULE(MODNAME, m) {
         .def_readwrite("sigcut", &IEpsSigCutAtom::sigcut)
         .def("max_size", &IEpsSigCutAtom::max_size);
 
+    py::class_<EpsSigAtom, AtomID>(m, "EpsSigAtom")
+        .def(py::init<AtomID, flt, flt>())
+        .def_readwrite("epsilon", &EpsSigAtom::epsilon)
+        .def_readwrite("sigma", &EpsSigAtom::sigma)
+        .def("max_size", &EpsSigAtom::max_size);
+    py::class_<EpsSigCutAtom, EpsSigAtom>(m, "EpsSigCutAtom")
+        .def(py::init<AtomID, flt, flt, flt>())
+        .def_readwrite("sigcut", &EpsSigCutAtom::sigcut)
+        .def("max_size", &EpsSigCutAtom::max_size);
+    py::class_<IEpsISigCutAtom, AtomID>(m, "IEpsISigCutAtom")
+        .def(py::init<AtomID, vector<flt>, vector<flt>, uint, flt>())
+        .def_readwrite("epsilons", &IEpsISigCutAtom::epsilons)
+        .def_readwrite("sigmas", &IEpsISigCutAtom::sigmas)
+        .def_readwrite("indx", &IEpsISigCutAtom::indx)
+        .def_readwrite("sigcut", &IEpsISigCutAtom::sigcut)
+        .def("max_size", &IEpsISigCutAtom::max_size);
+    py::class_<IEpsISigExpAtom, AtomID>(m, "IEpsISigExpAtom")
+        .def(py::init<AtomID, vector<flt>, vector<flt>, uint, flt>(), py::arg("a"),
+             py::arg("epsilons"), py::arg("sigmas"), py::arg("indx"),
+             py::arg("exponent") = 2.5)
+        .def_readwrite("epsilons", &IEpsISigExpAtom::epsilons)
+        .def_readwrite("sigmas", &IEpsISigExpAtom::sigmas)
+        .def_readwrite("indx", &IEpsISigExpAtom::indx)
+        .def_readwrite("exponent", &IEpsISigExpAtom::exponent)
+        .def("max_size", &IEpsISigExpAtom::max_size);
+
     bind_nlisted<Repulsion, EpsSigExpAtom>(m, "Repulsion");
     bind_nlisted<LJAttractRepulse, IEpsSigCutAtom>(m, "LJAttractRepulse");
+    bind_nlisted<LJRepulse, EpsSigAtom>(m, "LJRepulse");
+    bind_nlisted<LJIICut, IEpsISigCutAtom>(m, "LJIICut");
+    bind_nlisted<RepulsionII, IEpsISigExpAtom>(m, "RepulsionII");
 
     py::class_<Constraint, sptr<Constraint> >(m, "Constraint");
     py::class_<Collection, sptr<Collection> >(m, "Collection")

@@ -72,12 +72,21 @@ def test_one_list_per_atomvec_on_device_path():
     d3.NeighborList(box, av, 0.3)  # released with its owner
 
 
-def test_exclusions_fail_loudly():
+def test_ignore_size_counts_unordered_pairs_once():
+    """NeighborList::ignore_size (src/trackers.hpp:204) counts what
+    PairList::add_pair stored (src/trackers.hpp:76-83): an unordered pair
+    once, whichever order it was given in; a pair of an atom with itself is
+    stored too.  Host-side bookkeeping, no device call."""
     box = d3.OriginBox(10.0)
-    av = d3.AtomVec([1.0] * 3)
+    av = d3.AtomVec([1.0] * 4)
     nl = d3.NeighborList(box, av, 0.3)
-    with pytest.raises(RuntimeError, match="ignore"):
-        nl.ignore(av[0], av[1])
+    assert nl.ignore_size() == 0
+    nl.ignore(av[0], av[1])
+    nl.ignore(av[1], av[0])
+    assert nl.ignore_size() == 1
+    nl.ignore(av[2], av[2])
+    nl.ignore(av[3], av[1])
+    assert nl.ignore_size() == 3
 
 
 def test_device_calls_raise_without_gpu():
