@@ -269,9 +269,11 @@ int parm_nlcg_destroy(parm_nlcg *c);
  * 1e-3, 0). */
 int parm_nlcg_set_limits(parm_nlcg *c, double alphamax, double afrac, double dxmax,
                          double stepmax, double maxdV);
-/* set_dt + set_pressure_goal + set_kappa (each ends in reset(),
- * src/collection.hpp:458-472). */
-int parm_nlcg_set_params(parm_nlcg *c, double dt, double P0, double kappa);
+/* The parameter members dt, P0, kappa, kmax, secmax, seceps
+ * (src/collection.hpp:408-421).  The reference's set_dt / set_pressure_goal /
+ * set_kappa (:458-472) also call reset(): call parm_nlcg_reset after. */
+int parm_nlcg_set_params(parm_nlcg *c, double dt, double P0, double kappa, double kmax,
+                         int secmax, double seceps);
 /* reset(), set_forces(constraints_and_a, setV), timestep() x nsteps and
  * descend() (src/collection.cpp:528-569, 685-853). */
 int parm_nlcg_reset(parm_nlcg *c);

@@ -1970,14 +1970,18 @@ int parm_nlcg_reset(parm_nlcg *c) {
     return PARM_OK;
 }
 
-int parm_nlcg_set_params(parm_nlcg *c, double dt, double P0, double kappa) {
+int parm_nlcg_set_params(parm_nlcg *c, double dt, double P0, double kappa, double kmax,
+                         int secmax, double seceps) {
     PARM_CHECK(check_nlcg(c));
     if (!(kappa > 0)) return fail(PARM_E_INVALID, "kappa must be > 0");
-    // set_dt / set_pressure_goal / set_kappa each end in reset()
+    if (secmax < 0) return fail(PARM_E_INVALID, "secmax must be >= 0");
     c->dt = dt;
     c->P0 = P0;
     c->kappa = kappa;
-    return parm_nlcg_reset(c);
+    c->kmax = kmax;
+    c->secmax = (unsigned)secmax;
+    c->seceps = seceps;
+    return PARM_OK;
 }
 
 int parm_nlcg_set_forces(parm_nlcg *c, int constraints_and_a, int setV) {

@@ -80,6 +80,17 @@ class OriginBox : public Box {
         return diff(v, Vec::Zero());
     }
     Vec box_shape() { return boxsize; }
+    //! src/box.cpp:3-70 (box only; the atoms are not scaled)
+    flt resize(flt factor) {
+        boxsize *= factor;
+        return V();
+    }
+    flt resize_to(Vec newsize) {
+        boxsize = newsize;
+        return V();
+    }
+    flt resize_to_V(flt newV) { return resize(std::pow(newV / V(), OVERNDIM)); }
+    flt resize_to_L(flt newL) { return resize(newL / std::pow(V(), OVERNDIM)); }
 };
 
 struct Atom {
@@ -419,6 +430,30 @@ class AtomVec : public virtual AtomGroup {
         dev_->precision = precision;
     }
     DeviceState &device() const { return *dev_; }
+
+    //! Device-path extension (no reference counterpart): queues an
+    //! asynchronous copy of the device positions (and velocities) for output
+    //! and returns its ticket; the copy travels while later steps run.
+    int64_t snapshot(bool positions = true, bool velocities = false) const {
+        dev_->ensure();
+        int64_t t = 0;
+        parm_check(parm_snapshot_begin(dev_->sys, (positions ? 1 : 0) | (velocities ? 2 : 0),
+                                       &t));
+        return t;
+    }
+    //! Writes snapshot `ticket` as one xyz frame (the format of the
+    //! reference's drivers): elements holds one letter per atom, or is empty
+    //! for all 'C'; precision as an ostream's (6 by default).
+    void write_xyz(int64_t ticket, const std::string &path, bool append = true,
+                   const std::string &comment = "", const std::string &elements = "",
+                   int precision = 6) const {
+        if (!elements.empty() && elements.size() != sz)
+            throw std::invalid_argument("write_xyz: one element letter per atom");
+        parm_check(parm_snapshot_write_xyz(dev_->sys, ticket, path.c_str(), append ? 1 : 0,
+                                           comment.c_str(),
+                                           elements.empty() ? nullptr : elements.c_str(),
+                                           precision));
+    }
 
     ~AtomVec() { delete[] atoms; }
 };

@@ -248,5 +248,176 @@ class CollectionDamped : public Collection {
     }
 };
 
+//! Pressure-controlled nonlinear conjugate-gradient energy minimiser
+//! (src/collection.hpp:400-474, src/collection.cpp:494-855), minimising
+//! H = U + P0 V over the positions and the box: the device runs it
+//! (parm_nlcg_*); the public members mirror the reference's and are
+//! exchanged with the device around every call.
+class CollectionNLCG : public Collection {
+   public:
+    // parameters
+    flt dt;
+    flt seceps;
+    uint secmax;
+    flt kappa;
+    flt alphamax, afrac, dxmax, stepmax, kmax;
+    flt P0;
+    // kept between iterations
+    flt Knew, k, vl, fl, al;
+    // for tracking
+    flt alpha, beta, betaused, dxsum, alphavmax, maxdV;
+    uint sec;
+
+   protected:
+    parm_nlcg *dev_c = nullptr;
+    bool fixedL = false;
+
+    OriginBox &obox() {
+        OriginBox *ob = dynamic_cast<OriginBox *>(box.get());
+        if (!ob) throw std::runtime_error("CollectionNLCG needs an OriginBox");
+        return *ob;
+    }
+    //! The device minimiser, created on first use (the collection's
+    //! interaction and list are fixed from then on); members pushed.
+    parm_nlcg *device_nlcg() {
+        parm_system *s = device_step_system();
+        if (!dev_c)
+            parm_check(parm_nlcg_create(s, dt, P0, kappa, kmax, (int)secmax, seceps,
+                                        fixedL ? 1 : 0, &dev_c));
+        parm_check(parm_nlcg_set_params(dev_c, dt, P0, kappa, kmax, (int)secmax, seceps));
+        parm_check(parm_nlcg_set_limits(dev_c, alphamax, afrac, dxmax, stepmax, maxdV));
+        return dev_c;
+    }
+    //! Members and the box back from the device.
+    void pull() {
+        double o[12];
+        parm_check(parm_nlcg_info(dev_c, o));
+        alpha = o[0];
+        beta = o[1];
+        betaused = o[2];
+        dxsum = o[3];
+        alphavmax = o[4];
+        sec = (uint)o[5];
+        k = o[6];
+        vl = o[7];
+        fl = o[8];
+        al = o[9];
+        Knew = o[10];
+        DeviceState &d = atomvec().device();
+        Vec L;
+        parm_check(parm_get_box(d.sys, L.data()));
+        obox().resize_to(L);
+        d.box_L = L;  // the device already has it
+        d.dev_newer = true;
+    }
+
+   public:
+    CollectionNLCG(sptr<OriginBox> box, sptr<AtomGroup> atoms, const flt dt, const flt P0,
+                   vector<sptr<Interaction> > interactions = vector<sptr<Interaction> >(),
+                   vector<sptr<StateTracker> > trackers = vector<sptr<StateTracker> >(),
+                   vector<sptr<Constraint> > constraints = vector<sptr<Constraint> >(),
+                   const flt kappa = 10.0, const flt kmax = 1000, const uint secmax = 40,
+                   const flt seceps = 1e-20)
+        : Collection(box, atoms, interactions, trackers, constraints),
+          dt(dt),
+          seceps(seceps),
+          secmax(secmax),
+          kappa(kappa),
+          alphamax(2.0),
+          afrac(0),
+          dxmax(100),
+          stepmax(1e-3),
+          kmax(kmax),
+          P0(P0),
+          Knew(0),
+          k(0),
+          vl(0),
+          fl(0),
+          al(0),
+          alpha(0),
+          beta(0),
+          betaused(0),
+          dxsum(0),
+          alphavmax(0),
+          maxdV(0),
+          sec(0) {}
+    ~CollectionNLCG() {
+        if (dev_c) parm_nlcg_destroy(dev_c);
+    }
+
+    flt kinetic_energy() {
+        double K = 0;
+        parm_check(parm_nlcg_observables(device_nlcg(), nullptr, nullptr, &K));
+        return K;
+    }
+    flt pressure() {
+        double P = 0;
+        parm_check(parm_nlcg_observables(device_nlcg(), nullptr, &P, nullptr));
+        return P;
+    }
+    flt hamiltonian() {
+        double H = 0;
+        parm_check(parm_nlcg_observables(device_nlcg(), &H, nullptr, nullptr));
+        return H;
+    }
+    void set_forces(bool constraints_and_a = true) { set_forces(constraints_and_a, true); }
+    void set_forces(bool constraints_and_a, bool setV) {
+        if (!device_interaction()) {
+            Collection::set_forces(constraints_and_a);
+            return;
+        }
+        parm_check(parm_nlcg_set_forces(device_nlcg(), constraints_and_a, setV));
+        pull();
+    }
+    void timestep() { timesteps(1); }
+    //! Device-path extension: n timesteps in one call.
+    void timesteps(uint64_t n) {
+        parm_check(parm_nlcg_timestep(device_nlcg(), (int64_t)n));
+        pull();
+    }
+    void descend() {
+        parm_check(parm_nlcg_descend(device_nlcg()));
+        pull();
+    }
+    void reset() {
+        parm_check(parm_nlcg_reset(device_nlcg()));
+        pull();
+    }
+    void set_dt(flt newdt) {
+        dt = newdt;
+        reset();
+    }
+    void set_pressure_goal(flt P) {
+        P0 = P;
+        reset();
+    }
+    flt get_pressure_goal() { return P0; }
+    void set_kappa(flt kk) {
+        kappa = kk;
+        reset();
+    }
+    void set_max_alpha(flt a) { alphamax = a; }
+    void set_max_alpha_fraction(flt a) { afrac = a; }
+    void set_max_dx(flt d) { dxmax = d; }
+    void set_max_step(flt m) { stepmax = m; }
+};
+
+//! Like CollectionNLCG, but only the first box edge changes
+//! (src/collection.hpp:476-488, src/collection.cpp:855-870).
+class CollectionNLCGFixedL : public CollectionNLCG {
+   public:
+    CollectionNLCGFixedL(
+        sptr<OriginBox> box, sptr<AtomGroup> atoms, const flt dt, const flt P0,
+        vector<sptr<Interaction> > interactions = vector<sptr<Interaction> >(),
+        vector<sptr<StateTracker> > trackers = vector<sptr<StateTracker> >(),
+        vector<sptr<Constraint> > constraints = vector<sptr<Constraint> >(),
+        const flt kappa = 10.0, const flt kmax = 1000, const uint secmax = 40,
+        const flt seceps = 1e-20)
+        : CollectionNLCG(box, atoms, dt, P0, interactions, trackers, constraints, kappa, kmax,
+                         secmax, seceps) {
+        fixedL = true;
+    }
+};
+
 PARM_NS_END
 #endif

@@ -92,6 +92,32 @@ static void set_field(AtomVec &av, int which,
     }
 }
 
+// Atom state as the reference's pickling stores it (src/sim.i:494-503):
+// (tuple(x), tuple(v), tuple(f), tuple(a)).
+static py::tuple vec_tuple(const Vec &v) {
+    py::tuple t(NDIM);
+    for (int k = 0; k < NDIM; k++) t[k] = v[k];
+    return t;
+}
+static Vec tuple_vec(py::handle h) {
+    py::sequence q = py::reinterpret_borrow<py::sequence>(h);
+    if (py::len(q) != NDIM) throw std::invalid_argument("state vector of the wrong length");
+    Vec v;
+    for (int k = 0; k < NDIM; k++) v[k] = q[k].cast<double>();
+    return v;
+}
+static py::tuple atom_state(const Atom &a) {
+    return py::make_tuple(vec_tuple(a.x), vec_tuple(a.v), vec_tuple(a.f), vec_tuple(a.a));
+}
+static void set_atom_state(Atom &a, py::handle st) {
+    py::sequence q = py::reinterpret_borrow<py::sequence>(st);
+    if (py::len(q) != 4) throw std::invalid_argument("an atom state holds x, v, f, a");
+    a.x = tuple_vec(q[0]);
+    a.v = tuple_vec(q[1]);
+    a.f = tuple_vec(q[2]);
+    a.a = tuple_vec(q[3]);
+}
+
 template <class NL, class A>
 static void bind_nlisted(py::module_ &m, const char *name) {
     py::class_<NL, Interaction, sptr<NL> >(m, name)
@@ -142,11 +168,24 @@ PYBIND11_MODULE(MODNAME, m) {
         .def("box_shape", &OriginBox::box_shape);
 
     py::class_<Atom>(m, "Atom")
+        .def(py::init([]() {
+            Atom a;
+            a.x = a.v = a.a = a.f = Vec::Zero();
+            a.m = 0;
+            return a;
+        }))
         .def_readwrite("x", &Atom::x)
         .def_readwrite("v", &Atom::v)
         .def_readwrite("a", &Atom::a)
         .def_readwrite("f", &Atom::f)
-        .def_readwrite("m", &Atom::m);
+        .def_readwrite("m", &Atom::m)
+        .def(py::pickle([](const Atom &a) { return atom_state(a); },
+                        [](py::tuple st) {
+                            Atom a;
+                            a.m = 0;
+                            set_atom_state(a, st);
+                            return a;
+                        }));
 
     py::class_<AtomID>(m, "AtomID")
         .def("n", &AtomID::n)
@@ -166,6 +205,7 @@ PYBIND11_MODULE(MODNAME, m) {
         .def_property(
             "m", [](const AtomID &a) { return a.peek().m; },
             [](AtomID &a, flt m) { a->m = m; })
+        .def("__getstate__", [](const AtomID &a) { return atom_state(a.peek()); })
         .def("__eq__", [](const AtomID &a, const AtomID &b) { return a == b; })
         .def("__hash__", [](const AtomID &a) { return a.n(); })
         .def("__repr__", [](const AtomID &a) {
@@ -229,7 +269,34 @@ PYBIND11_MODULE(MODNAME, m) {
              [](AtomVec &av, py::array_t<double> a) { set_field(av, 1, a); })
         .def("set_accelerations",
              [](AtomVec &av, py::array_t<double> a) { set_field(av, 2, a); })
-        .def("set_forces", [](AtomVec &av, py::array_t<double> a) { set_field(av, 3, a); });
+        .def("set_forces", [](AtomVec &av, py::array_t<double> a) { set_field(av, 3, a); })
+        .def("snapshot", &AtomVec::snapshot, py::arg("positions") = true,
+             py::arg("velocities") = false)
+        .def("write_xyz", &AtomVec::write_xyz, py::arg("ticket"), py::arg("path"),
+             py::arg("append") = true, py::arg("comment") = "", py::arg("elements") = "",
+             py::arg("precision") = 6)
+        // (masses, [atom states]) as the reference pickles an AtomVec
+        // (src/sim.i:546-576)
+        .def(py::pickle(
+            [](AtomVec &av) {
+                py::list masses, states;
+                for (uint i = 0; i < av.size(); i++) {
+                    const Atom &a = av.get_id(i).peek();
+                    masses.append(a.m);
+                    states.append(atom_state(a));
+                }
+                return py::make_tuple(masses, states);
+            },
+            [](py::tuple st) {
+                if (py::len(st) != 2) throw std::invalid_argument("AtomVec state");
+                vector<double> masses = st[0].cast<vector<double> >();
+                py::sequence states = st[1].cast<py::sequence>();
+                if (py::len(states) != masses.size())
+                    throw std::invalid_argument("AtomVec state: one state per mass");
+                auto av = std::make_shared<AtomVec>(masses);
+                for (uint i = 0; i < masses.size(); i++) set_atom_state((*av)[i], states[i]);
+                return av;
+            }));
 
     py::class_<StateTracker, sptr<StateTracker> >(m, "StateTracker")
         .def("update", &StateTracker::update);
@@ -352,4 +419,63 @@ PYBIND11_MODULE(MODNAME, m) {
         .def("timesteps", &CollectionDamped::timesteps)
         .def("change_damping", &CollectionDamped::change_damping)
         .def("set_dt", &CollectionDamped::set_dt);
+    py::class_<CollectionNLCG, Collection, sptr<CollectionNLCG> >(m, "CollectionNLCG")
+        .def(py::init<sptr<OriginBox>, sptr<AtomGroup>, flt, flt, vector<sptr<Interaction> >,
+                      vector<sptr<StateTracker> >, vector<sptr<Constraint> >, flt, flt, uint,
+                      flt>(),
+             py::arg("box"), py::arg("atoms"), py::arg("dt"), py::arg("P0"),
+             py::arg("interactions") = vector<sptr<Interaction> >(),
+             py::arg("trackers") = vector<sptr<StateTracker> >(),
+             py::arg("constraints") = vector<sptr<Constraint> >(), py::arg("kappa") = 10.0,
+             py::arg("kmax") = 1000.0, py::arg("secmax") = 40u, py::arg("seceps") = 1e-20)
+        .def("timestep", &CollectionNLCG::timestep)
+        .def("timesteps", &CollectionNLCG::timesteps)
+        .def("descend", &CollectionNLCG::descend)
+        .def("reset", &CollectionNLCG::reset)
+        .def("set_forces",
+             py::overload_cast<bool, bool>(&CollectionNLCG::set_forces),
+             py::arg("constraints_and_a") = true, py::arg("setV") = true)
+        .def("kinetic_energy", &CollectionNLCG::kinetic_energy)
+        .def("pressure", &CollectionNLCG::pressure)
+        .def("hamiltonian", &CollectionNLCG::hamiltonian)
+        .def("set_dt", &CollectionNLCG::set_dt)
+        .def("set_pressure_goal", &CollectionNLCG::set_pressure_goal)
+        .def("get_pressure_goal", &CollectionNLCG::get_pressure_goal)
+        .def("set_kappa", &CollectionNLCG::set_kappa)
+        .def("set_max_alpha", &CollectionNLCG::set_max_alpha)
+        .def("set_max_alpha_fraction", &CollectionNLCG::set_max_alpha_fraction)
+        .def("set_max_dx", &CollectionNLCG::set_max_dx)
+        .def("set_max_step", &CollectionNLCG::set_max_step)
+        .def_readwrite("dt", &CollectionNLCG::dt)
+        .def_readwrite("seceps", &CollectionNLCG::seceps)
+        .def_readwrite("secmax", &CollectionNLCG::secmax)
+        .def_readwrite("kappa", &CollectionNLCG::kappa)
+        .def_readwrite("alphamax", &CollectionNLCG::alphamax)
+        .def_readwrite("afrac", &CollectionNLCG::afrac)
+        .def_readwrite("dxmax", &CollectionNLCG::dxmax)
+        .def_readwrite("stepmax", &CollectionNLCG::stepmax)
+        .def_readwrite("kmax", &CollectionNLCG::kmax)
+        .def_readwrite("P0", &CollectionNLCG::P0)
+        .def_readwrite("maxdV", &CollectionNLCG::maxdV)
+        .def_readonly("Knew", &CollectionNLCG::Knew)
+        .def_readonly("k", &CollectionNLCG::k)
+        .def_readonly("vl", &CollectionNLCG::vl)
+        .def_readonly("fl", &CollectionNLCG::fl)
+        .def_readonly("al", &CollectionNLCG::al)
+        .def_readonly("alpha", &CollectionNLCG::alpha)
+        .def_readonly("beta", &CollectionNLCG::beta)
+        .def_readonly("betaused", &CollectionNLCG::betaused)
+        .def_readonly("dxsum", &CollectionNLCG::dxsum)
+        .def_readonly("alphavmax", &CollectionNLCG::alphavmax)
+        .def_readonly("sec", &CollectionNLCG::sec);
+    py::class_<CollectionNLCGFixedL, CollectionNLCG, sptr<CollectionNLCGFixedL> >(
+        m, "CollectionNLCGFixedL")
+        .def(py::init<sptr<OriginBox>, sptr<AtomGroup>, flt, flt, vector<sptr<Interaction> >,
+                      vector<sptr<StateTracker> >, vector<sptr<Constraint> >, flt, flt, uint,
+                      flt>(),
+             py::arg("box"), py::arg("atoms"), py::arg("dt"), py::arg("P0"),
+             py::arg("interactions") = vector<sptr<Interaction> >(),
+             py::arg("trackers") = vector<sptr<StateTracker> >(),
+             py::arg("constraints") = vector<sptr<Constraint> >(), py::arg("kappa") = 10.0,
+             py::arg("kmax") = 1000.0, py::arg("secmax") = 40u, py::arg("seceps") = 1e-20);
 }

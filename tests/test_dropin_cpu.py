@@ -103,3 +103,31 @@ def test_device_calls_raise_without_gpu():
         d3.CollectionVerlet(box, av, 0.01, [r], [r.neighbor_list()])
     with pytest.raises(ValueError):
         d3.CollectionDamped(box, av, -1.0, 1.0, [r], [r.neighbor_list()])
+
+
+def test_atomvec_and_atom_pickle_round_trip():
+    """Pickled state as the reference's SWIG extension stores it
+    (src/sim.i:494-576): an Atom as (x, v, f, a); an AtomVec as
+    (masses, [atom states]).  Host bookkeeping only, no device call."""
+    import pickle
+
+    av = d3.AtomVec([1.0, 2.0, 0.5])
+    for i, a in enumerate(av):
+        a.x = [i, 2.0 * i, -i]
+        a.v = [0.5, i, 0.0]
+        a.f = [1.0, 1.0, i]
+        a.a = [0.0, -1.0, i]
+    masses, states = av.__getstate__()
+    assert masses == [1.0, 2.0, 0.5]
+    assert states[1] == ((1.0, 2.0, -1.0), (0.5, 1.0, 0.0), (1.0, 1.0, 1.0), (0.0, -1.0, 1.0))
+    assert av[2].__getstate__() == states[2]
+    av2 = pickle.loads(pickle.dumps(av))
+    assert av2.size() == 3 and [av2.getmass(i) for i in range(3)] == masses
+    assert av2.__getstate__() == (masses, states)
+    at = d3.Atom()
+    at.x, at.v, at.f, at.a = [1, 2, 3], [4, 5, 6], [7, 8, 9], [0, 1, 0]
+    at2 = pickle.loads(pickle.dumps(at))
+    assert np.array_equal(at2.x, [1, 2, 3]) and np.array_equal(at2.a, [0, 1, 0])
+    p2 = d2.AtomVec([1.0])
+    p2[0].x = [0.25, -0.5]
+    assert pickle.loads(pickle.dumps(p2)).get_positions().tolist() == [[0.25, -0.5]]

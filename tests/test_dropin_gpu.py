@@ -14,6 +14,7 @@ import pytest
 
 from oracle import oracle as O
 from parm_amd import capi, d2, d3
+from tests import systems as S
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -162,15 +163,107 @@ def test_dropin_damped_2d_relaxation():
     assert rep.overlaps(box) <= rep.contacts(box)
 
 
-def test_cpp_ljatoms_example_runs():
+def test_cpp_ljatoms_example_runs(tmp_path):
+    """The reference driver's shape (src/bin/LJatoms.cpp) on the device:
+    NListed<EpsSigCutAtom, LennardJonesCutPair>, random insertion with the
+    energy check, NVE Verlet, an xyz frame per outer step."""
     if capi.device_count() < 1:
         pytest.fail("no GPU visible")
     exe = os.path.join(ROOT, "parm_amd", "host", "examples", "LJatoms")
-    r = subprocess.run([exe, "4000", "5", "100"], capture_output=True, text=True,
-                       timeout=120)
+    xyz = tmp_path / "LJatoms.xyz"
+    r = subprocess.run([exe, "600", "5", "400", str(xyz)], capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stderr
     lines = [ln for ln in r.stdout.splitlines() if ln.startswith("step")]
     E = [float(ln.split()[3]) for ln in lines]
     E0 = float([ln for ln in r.stdout.splitlines() if ln.startswith("E0")][0].split()[1])
     assert len(E) == 5
     assert max(abs(e - E0) for e in E) < 1e-3 * abs(E0)
+    frames = xyz.read_text().split("\n")
+    assert len(frames) - 1 == 6 * 602 and frames[0] == "600"
+    assert all(ln.startswith("C\t") for ln in frames[2:602])
+
+
+def test_dropin_nlcg_packer_flow():
+    """packer.cpp's flow (src/bin/packer.cpp:60-147) through the drop-in
+    module: CollectionNLCG built first, tracker and interaction added after,
+    a pressure goal set (reset), timesteps until mechanical equilibrium at
+    the goal; the host OriginBox follows the device's resizing."""
+    if capi.device_count() < 1:
+        pytest.fail("no GPU visible")
+    n = 256
+    rng = np.random.default_rng(3)
+    sig = np.where(np.arange(n) < n // 2, 1.0, 1.4)
+    L0 = float(np.sqrt(np.sum(np.pi / 4 * sig ** 2) / 0.5))
+    box = d2.OriginBox(L0)
+    atoms = d2.AtomVec([1.0] * n)
+    hertz = d2.Repulsion(box, atoms, 0.1)
+    for a, sg in zip(atoms, sig):
+        a.x = rng.uniform(0, L0, 2)
+        hertz.add(d2.EpsSigExpAtom(a, 1.0, float(sg), 2.0))
+    nl = hertz.neighbor_list()
+    nl.update_list(True)
+    collec = d2.CollectionNLCG(box, atoms, 0.1, 1e-2)
+    collec.add_tracker(nl)
+    collec.add_interaction(hertz)
+    collec.set_max_step(1e-2)
+    collec.set_pressure_goal(1e-2)
+    H0 = collec.hamiltonian()
+    for _ in range(100):
+        collec.timesteps(100)
+        if abs(collec.pressure() / 1e-2 - 1) < 1e-9 and \
+                np.abs(atoms.get_forces()).max() < 1e-11:
+            break
+    assert abs(collec.pressure() / 1e-2 - 1) < 1e-9
+    assert np.abs(atoms.get_forces()).max() < 1e-11
+    assert collec.hamiltonian() < H0
+    assert box.V() != L0 * L0
+    assert collec.get_pressure_goal() == 1e-2 and collec.k >= 0
+
+
+def test_dropin_further_pair_kinds_and_ignore():
+    """LJRepulse, LJIICut and RepulsionII through the drop-in module against
+    the oracle's energies, and NeighborList.ignore removing pairs."""
+    if capi.device_count() < 1:
+        pytest.fail("no GPU visible")
+    n = 800
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=12, jitter=0.1)
+    types = (np.arange(n) % 2).astype(np.int32)
+    cases = [
+        ("LJRepulse", lambda a, t: d3.EpsSigAtom(a, [1.0, 0.5][t], [1.0, 1.2][t]),
+         O.PairTable(O.LJ_REPULSE, [1.0, 0.5], [1.0, 1.2], None, types),
+         np.array([1.0, 1.2])[types]),
+        ("LJIICut", lambda a, t: d3.IEpsISigCutAtom(a, [[1.0, 0.7], [0.7, 0.9]][t],
+                                                    [[1.0, 1.1], [1.1, 1.2]][t], t, 2.5),
+         O.PairTable(O.LJ_CUT_II, [[1.0, 0.7], [0.7, 0.9]], [[1.0, 1.1], [1.1, 1.2]],
+                     [2.5, 2.5], types),
+         np.array([1.1 * 2.5, 1.2 * 2.5])[types]),
+        ("RepulsionII", lambda a, t: d3.IEpsISigExpAtom(a, [[1.0, 0.8], [0.8, 1.2]][t],
+                                                        [[1.0, 1.2], [1.2, 1.4]][t], t, 2.0),
+         O.PairTable(O.REPULSION_II, [[1.0, 0.8], [0.8, 1.2]], [[1.0, 1.2], [1.2, 1.4]],
+                     [2.0, 2.0], types),
+         np.array([1.2, 1.4])[types]),
+    ]
+    for name, make_atom, table, diam in cases:
+        print(f"drop-in pair kind {name}", flush=True)
+        box = d3.OriginBox(float(L[0]))
+        atoms = d3.AtomVec([1.0] * n)
+        inter = getattr(d3, name)(box, atoms, 0.3)
+        atoms.set_positions(x)
+        for a, t in zip(atoms, types):
+            inter.add(make_atom(a, int(t)))
+        nl = inter.neighbor_list()
+        nl.update_list(True)
+        first, last = O.nlist(x, L, diam, 0.3)
+        assert nl.numpairs() == len(first)
+        ref = O.forces(x, L, table, first, last)
+        assert abs(inter.energy(box) - ref["U"]) <= 1e-12 * abs(ref["U"])
+        # ignore the first listed pair: the next update drops it
+        p0 = nl[0]
+        nl.ignore(p0.first(), p0.last())
+        assert nl.ignore_size() == 1
+        assert nl.update_list(False)
+        f2, l2 = O.nlist(x, L, diam, 0.3, ignore=[(p0.first().n(), p0.last().n())])
+        assert nl.numpairs() == len(f2) == len(first) - 1
+        got = [(p.first().n(), p.last().n()) for p in nl]
+        assert got == list(zip(f2.tolist(), l2.tolist()))

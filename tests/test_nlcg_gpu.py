@@ -68,11 +68,11 @@ def test_nlcg_follows_the_oracle(ndim, n, fixed):
 @pytest.mark.parametrize("ndim", [2, 3])
 def test_nlcg_reaches_equilibrium_on_device(ndim):
     _device_or_skip()
-    s, _ = _packing(2000 if ndim == 2 else 3000, ndim, 0.55, seed=ndim)
+    s, _ = _packing(300 if ndim == 2 else 400, ndim, 0.55, seed=ndim)
     P0 = 1e-2
     c = capi.NLCG(s, 0.1, P0)
     c.set_limits(stepmax=1e-2)
-    for _ in range(60):
+    for _ in range(100):
         c.timestep(100)
         fmax = s.force_stats()[0]
         if fmax < 1e-11 and abs(c.observables()["P"] / P0 - 1) < 1e-9:
