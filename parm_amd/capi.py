@@ -123,7 +123,7 @@ def lib():
                                        C.POINTER(_vp)]),
             "parm_nlcg_destroy": (i32, [_vp]),
             "parm_nlcg_set_limits": (i32, [_vp, dbl, dbl, dbl, dbl, dbl]),
-            "parm_nlcg_set_params": (i32, [_vp, dbl, dbl, dbl]),
+            "parm_nlcg_set_params": (i32, [_vp, dbl, dbl, dbl, dbl, i32, dbl]),
             "parm_nlcg_reset": (i32, [_vp]),
             "parm_nlcg_set_forces": (i32, [_vp, i32, i32]),
             "parm_nlcg_timestep": (i32, [_vp, i64]),
@@ -451,8 +451,11 @@ class NLCG:
         _check(lib().parm_nlcg_set_limits(self.h, float(alphamax), float(afrac),
                                           float(dxmax), float(stepmax), float(maxdV)))
 
-    def set_params(self, dt, P0, kappa):
-        _check(lib().parm_nlcg_set_params(self.h, float(dt), float(P0), float(kappa)))
+    def set_params(self, dt, P0, kappa, kmax=1000, secmax=40, seceps=1e-20):
+        """Sets the parameter members; like the reference's set_dt /
+        set_pressure_goal / set_kappa, follow with reset()."""
+        _check(lib().parm_nlcg_set_params(self.h, float(dt), float(P0), float(kappa),
+                                          float(kmax), int(secmax), float(seceps)))
 
     def reset(self):
         _check(lib().parm_nlcg_reset(self.h))

@@ -153,6 +153,7 @@ struct parm_system {
     int ntypes = 0;
     int single = 1, harmonic = 0;
     PairParam *tab = nullptr;
+    PairParam tab0{};  // host copy of tab[0]
 
     // reductions and flags
     double *part = nullptr, *out = nullptr;
@@ -212,6 +213,7 @@ struct parm_system {
         L.tiled = tiled ? 1 : 0;
         L.ntiles = ntiles;
         L.smax = smax;
+        L.wcap = tpa > 0 ? cap / tpa : 0;
         L.tiles = tiles;
         L.truns = truns;
         L.keys = keys;
@@ -226,6 +228,7 @@ struct parm_system {
         P.single = single;
         P.harmonic = harmonic;
         P.tab = tab;
+        P.p0 = tab0;
         return P;
     }
     ExclView exview() const {
@@ -986,6 +989,7 @@ int build_pair_table(parm_system *s, int form, int ntypes, const double *eps,
     PARM_CHECK(dalloc(&s->tab, tab.size()));
     PARM_HIP(hipMemcpy(s->tab, tab.data(), sizeof(PairParam) * tab.size(),
                        hipMemcpyHostToDevice));
+    s->tab0 = tab[0];
     if (s->n > 0) PARM_CHECK(upload_scalar<int32_t>(s, atype, s->type));
     int bits = 0;
     while ((1 << bits) < ntypes) bits++;

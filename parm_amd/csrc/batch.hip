@@ -8,12 +8,13 @@
 // the device per system, so a step needs no host round trip and a run of
 // steps can be captured in a HIP graph.
 //
-// Per step:  k_batch_force   forces + kick (+ next drift, |x - xref| per atom)
-//            k_batch_rebuild one workgroup per system; exits unless that
-//                            system's skin flag is set: cell sort in LDS,
-//                            sorted position copy, exact list build, xref = x
-//            k_batch_skin    one wavefront per system: top-2 of |x - xref|
-//                            -> the system's flag for the next rebuild
+// Per step:  k_batch_force    forces + kick (+ next drift, |x - xref| per atom)
+//            k_batch_worklist the systems whose skin flag is set, as a list
+//            k_batch_rebuild  a fixed grid taking the listed systems in turn:
+//                             cell sort in LDS, sorted positions staged in
+//                             LDS, exact list build, xref = x
+//            k_batch_skin     one wavefront per system: top-2 of |x - xref|
+//                             -> the system's flag for the next rebuild
 #include "common.hpp"
 #include "kernels.hpp"
 #include "pairs.hpp"
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
         const PairParam *row = P.tab;
         PairParam p0;
         if (SINGLE)
-            p0 = P.tab[0];
+            p0 = P.p0;
         else
             row = P.tab + (size_t)A.type[i] * P.ntypes;
         int q = sub;
@@ -233,33 +234,69 @@ __global__ __launch_bounds__(kBlock) void k_batch_skin(int nsys, BatchArrays A,
 }
 
 // -------------------------------------------------------------- rebuild
-// One workgroup of kRebuildThreads per system.  Exits unless the system's
-// flag is set (or force).  Cell keys and a stable counting sort of the
-// system's atoms by cell in LDS; the sorted positions and diameters are
-// copied to xs / dsort so each candidate is one contiguous load; then the
-// exact list build over the periodic 27-cell stencil (candidates in cell
-// order, then sorted order: deterministic), four candidates in flight per
-// thread; then xref = x.  With `alt` (the positions one step ahead),
-// |alt - xref| is recomputed for the next skin test.
+// The systems to rebuild at the end of a step are those whose skin flag is
+// set (or all, when forced).  k_batch_worklist turns the flags into a list
+// (system ids in ascending order, count first); k_batch_rebuild then runs a
+// fixed grid of workgroups that take the listed systems in turn, so the
+// launch shape does not depend on how many systems rebuild (the step stays
+// graph-capturable) and no workgroup is started for a system that keeps its
+// list.  Per system, one workgroup of kRebuildThreads: cell keys and a
+// stable counting sort of the system's atoms by cell in LDS; the sorted
+// positions and diameters are staged in LDS (systems of up to `stage_max`
+// atoms; larger ones are copied to xs / dsort in memory) so each candidate
+// is an LDS read; then the exact list build over the periodic 27-cell
+// stencil (candidates in cell order, then sorted order: deterministic), four
+// candidates in flight per thread; then xref = x.  With `alt` (the
+// positions one step ahead), |alt - xref| is recomputed for the next skin
+// test.
 constexpr int kBatchMaxCells = kBatchMaxCellsPerSystem;
 constexpr int kRebuildThreads = 1024;
+constexpr int kWorklistThreads = 1024;
 
-template <int D>
-__global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
-    BatchArrays A, const typename PosStore<D>::T *__restrict__ pos,
-    const typename PosStore<D>::T *__restrict__ alt, int force, int count_rebuild,
-    double skin, int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
-    int32_t *__restrict__ ncount, int32_t *__restrict__ keys,
-    int32_t *__restrict__ cellidx, int32_t *__restrict__ cstart,
-    typename PosStore<D>::T *__restrict__ xs, double *__restrict__ dsort,
-    uint32_t *__restrict__ flags, uint32_t *__restrict__ which,
-    unsigned long long *__restrict__ npairs, uint32_t *__restrict__ gflag) {
+__global__ __launch_bounds__(kWorklistThreads) void k_batch_worklist(
+    int nsys, const uint32_t *__restrict__ flags, int force, int32_t *__restrict__ work) {
+    __shared__ int wsum[kWorklistThreads / kWave];
+    const int t = threadIdx.x;
+    const int per = (nsys + kWorklistThreads - 1) / kWorklistThreads;
+    const int s0 = t * per;
+    int loc = 0;
+    for (int q = 0; q < per; q++) {
+        const int s = s0 + q;
+        loc += (s < nsys && (force || flags[s] != 0u)) ? 1 : 0;
+    }
+    const int lane = t % kWave, w = t / kWave;
+    int incl = loc;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int pos = incl - loc, tot = 0;
+    for (int q = 0; q < kWorklistThreads / kWave; q++) {
+        if (q < w) pos += wsum[q];
+        tot += wsum[q];
+    }
+    for (int q = 0; q < per; q++) {
+        const int s = s0 + q;
+        if (s < nsys && (force || flags[s] != 0u)) work[1 + pos++] = s;
+    }
+    if (t == 0) work[0] = tot;
+}
+
+template <int D, bool STAGED>
+__device__ __forceinline__ void rebuild_system(
+    const int s, const BatchArrays &A, const typename PosStore<D>::T *__restrict__ pos,
+    const typename PosStore<D>::T *__restrict__ alt, int count_rebuild, double skin,
+    int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
+    int32_t *__restrict__ ncount, int32_t *__restrict__ keys, int32_t *__restrict__ cellidx,
+    int32_t *__restrict__ cstart, typename PosStore<D>::T *__restrict__ xs,
+    double *__restrict__ dsort, uint32_t *__restrict__ flags, uint32_t *__restrict__ which,
+    unsigned long long *__restrict__ npairs, uint32_t *__restrict__ gflag, int *hist,
+    unsigned long long *red, double *lx, double *ld, int stage_max) {
     using PT = typename PosStore<D>::T;
     constexpr int NT = kRebuildThreads;
-    const int s = blockIdx.x;
-    if (!force && flags[s] == 0u) return;
-    __shared__ int hist[kBatchMaxCells];
-    __shared__ unsigned long long red[NT / kWave];
     const BatchGeom G = A.geom[s];
     const int off = G.off, n = G.n, nc0 = G.grid.nc[0], nc1 = G.grid.nc[1],
               nc2 = G.grid.nc[2], ncell = G.grid.ncell;
@@ -348,24 +385,48 @@ __global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
         }
     }
     // hist[c] is now the end of cell c (system-local sorted index).  Sorted
-    // copies of the positions and diameters, and xref = x (the reference
-    // stores lastlocs inside the rebuild loop).
+    // copies of the positions and diameters (in LDS when staged), and
+    // xref = x (the reference stores lastlocs inside the rebuild loop).
     for (int u = t; u < n; u += NT) {
         const int i = cellidx[off + u];
-        xs[off + u] = pos[i];
-        dsort[off + u] = A.diam[i];
+        const PT p = pos[i];
+        if constexpr (STAGED) {
+            const Vec<D> x = PosStore<D>::load(p);
+#pragma unroll
+            for (int k = 0; k < D; k++) lx[k * stage_max + u] = x.c[k];
+            ld[u] = A.diam[i];
+        } else {
+            xs[off + u] = p;
+            dsort[off + u] = A.diam[i];
+        }
     }
     for (int k = t; k < n; k += NT) static_cast<PT *>(A.xref_w)[off + k] = pos[off + k];
     __threadfence_block();
     __syncthreads();
+    auto cand_x = [&](int v) {
+        if constexpr (STAGED) {
+            Vec<D> x;
+#pragma unroll
+            for (int k = 0; k < D; k++) x.c[k] = lx[k * stage_max + v];
+            return x;
+        } else {
+            return PosStore<D>::load(xs[off + v]);
+        }
+    };
+    auto cand_d = [&](int v) {
+        if constexpr (STAGED)
+            return ld[v];
+        else
+            return dsort[off + v];
+    };
     // list build: thread per atom in sorted order
     unsigned long long mycnt = 0;
     const int lo0 = nc0 >= 3 ? -1 : 0, lo1 = (D >= 2 && nc1 >= 3) ? -1 : 0,
               lo2 = (D == 3 && nc2 >= 3) ? -1 : 0;
     for (int u = t; u < n; u += NT) {
         const int i = cellidx[off + u];
-        const Vec<D> xi = PosStore<D>::load(xs[off + u]);
-        const double di = dsort[off + u];
+        const Vec<D> xi = cand_x(u);
+        const double di = cand_d(u);
         int key = keys[i];
         int ci[3] = {0, 0, 0};
         ci[0] = key % nc0;
@@ -392,8 +453,8 @@ __global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
                             const int v = v0 + q < s1 ? v0 + q : s0;
-                            xj[q] = PosStore<D>::load(xs[off + v]);
-                            dj[q] = dsort[off + v];
+                            xj[q] = cand_x(v);
+                            dj[q] = cand_d(v);
                         }
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
@@ -448,6 +509,38 @@ __global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
         npairs[s] = tot / 2;
         if (count_rebuild) which[s] += 1u;
         flags[s] = 0u;
+    }
+    // the next listed system reuses the LDS
+    __syncthreads();
+}
+
+template <int D>
+__global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
+    BatchArrays A, const typename PosStore<D>::T *__restrict__ pos,
+    const typename PosStore<D>::T *__restrict__ alt, int count_rebuild, double skin,
+    int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
+    int32_t *__restrict__ ncount, int32_t *__restrict__ keys, int32_t *__restrict__ cellidx,
+    int32_t *__restrict__ cstart, typename PosStore<D>::T *__restrict__ xs,
+    double *__restrict__ dsort, uint32_t *__restrict__ flags, uint32_t *__restrict__ which,
+    unsigned long long *__restrict__ npairs, uint32_t *__restrict__ gflag,
+    const int32_t *__restrict__ work, int stage_max) {
+    __shared__ int hist[kBatchMaxCells];
+    __shared__ unsigned long long red[kRebuildThreads / kWave];
+    extern __shared__ double lstage[];
+    double *lx = lstage;                   // D x stage_max positions (SoA)
+    double *ld = lstage + D * stage_max;   // stage_max diameters
+    const int count = work[0];
+    for (int q = blockIdx.x; q < count; q += gridDim.x) {
+        const int s = work[1 + q];
+        if (A.geom[s].n <= stage_max)
+            rebuild_system<D, true>(s, A, pos, alt, count_rebuild, skin, type_bits, tpa, cap,
+                                    npad, nbr, ncount, keys, cellidx, cstart, xs, dsort, flags,
+                                    which, npairs, gflag, hist, red, lx, ld, stage_max);
+        else
+            rebuild_system<D, false>(s, A, pos, alt, count_rebuild, skin, type_bits, tpa, cap,
+                                     npad, nbr, ncount, keys, cellidx, cstart, xs, dsort,
+                                     flags, which, npairs, gflag, hist, red, lx, ld,
+                                     stage_max);
     }
 }
 
@@ -623,24 +716,38 @@ hipError_t launch_batch_skin(int nsys, const BatchArrays &A, double skin, uint32
     return hipGetLastError();
 }
 
+size_t batch_rebuild_lds(int ndim, int stage_max) {
+    return (size_t)(ndim + 1) * (size_t)stage_max * sizeof(double);
+}
+
+size_t batch_rebuild_static_lds() {
+    return sizeof(int) * kBatchMaxCells + sizeof(unsigned long long) * (kRebuildThreads / kWave) +
+           sizeof(int) * (kRebuildThreads / kWave);
+}
+
 hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const void *pos,
                                 const void *alt, int force, int count_rebuild, double skin,
                                 int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                                 int32_t *ncount, int32_t *keys, int32_t *cellidx,
                                 int32_t *cstart, void *xs, double *dsort, uint32_t *flags,
                                 uint32_t *which, unsigned long long *npairs, uint32_t *gflag,
-                                hipStream_t s) {
+                                int32_t *work, int grid, int stage_max, hipStream_t s) {
     if (nsys <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_worklist, dim3(1), dim3(kWorklistThreads), 0, s, nsys, flags,
+                       force, work);
+    const int g = grid < nsys ? grid : nsys;
+    // systems of up to stage_max atoms build from LDS
+    const size_t lds = batch_rebuild_lds(ndim, stage_max);
     if (ndim == 3)
-        hipLaunchKernelGGL(k_batch_rebuild<3>, dim3(nsys), dim3(kRebuildThreads), 0, s, A,
-                           (const double4 *)pos, (const double4 *)alt, force, count_rebuild,
-                           skin, type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx,
-                           cstart, (double4 *)xs, dsort, flags, which, npairs, gflag);
+        hipLaunchKernelGGL(k_batch_rebuild<3>, dim3(g), dim3(kRebuildThreads), lds, s, A,
+                           (const double4 *)pos, (const double4 *)alt, count_rebuild, skin,
+                           type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx, cstart,
+                           (double4 *)xs, dsort, flags, which, npairs, gflag, work, stage_max);
     else
-        hipLaunchKernelGGL(k_batch_rebuild<2>, dim3(nsys), dim3(kRebuildThreads), 0, s, A,
-                           (const double2 *)pos, (const double2 *)alt, force, count_rebuild,
-                           skin, type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx,
-                           cstart, (double2 *)xs, dsort, flags, which, npairs, gflag);
+        hipLaunchKernelGGL(k_batch_rebuild<2>, dim3(g), dim3(kRebuildThreads), lds, s, A,
+                           (const double2 *)pos, (const double2 *)alt, count_rebuild, skin,
+                           type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx, cstart,
+                           (double2 *)xs, dsort, flags, which, npairs, gflag, work, stage_max);
     return hipGetLastError();
 }
 

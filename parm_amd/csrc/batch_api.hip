@@ -85,10 +85,14 @@ struct parm_batch {
     void *xs = nullptr;        // positions in each system's cell order (rebuild scratch)
     double *dsort = nullptr;   // list diameters in the same order
     uint32_t *flags = nullptr, *which = nullptr, *gflag = nullptr, *hgflag = nullptr;
+    int32_t *work = nullptr;   // worklist of systems to rebuild (count first)
+    int rgrid = 1;             // workgroups of the rebuild launch
+    int stage_max = 0;         // systems this small rebuild from LDS
     unsigned long long *npairs = nullptr;
 
     int pair_kind = kPairNone, ntypes = 0, single = 1, harmonic = 0;
     PairParam *tab = nullptr;
+    PairParam tab0{};  // host copy of tab[0]
     double *eatom = nullptr, *sysout = nullptr;  // 4 per atom; 4 x nsys sums
 
     hipGraphExec_t gexec = nullptr;
@@ -139,6 +143,7 @@ struct parm_batch {
         P.single = single;
         P.harmonic = harmonic;
         P.tab = tab;
+        P.p0 = tab0;
         return P;
     }
 };
@@ -173,6 +178,7 @@ void bdestroy(parm_batch *b) {
     bfree(b->dsort);
     bfree(b->cstart);
     bfree(b->flags);
+    bfree(b->work);
     bfree(b->which);
     bfree(b->gflag);
     bfree(b->npairs);
@@ -277,7 +283,7 @@ int brebuild(parm_batch *b, bool force, bool count, const void *pos,
     PB_HIP(launch_batch_rebuild(b->ndim, b->nsys, b->arrays(), pos, alt, force ? 1 : 0,
                                 count ? 1 : 0, b->skin, b->type_bits, b->tpa, b->cap,
                                 b->npad, b->nbr, b->ncount, b->keys, b->cellidx, b->cstart,
-                                b->xs, b->dsort, b->flags, b->which, b->npairs, b->gflag,
+                                b->xs, b->dsort, b->flags, b->which, b->npairs, b->gflag, b->work, b->rgrid, b->stage_max,
                                 b->stream));
     return PARM_OK;
 }
@@ -512,6 +518,18 @@ int parm_batch_create(int ndim, int precision, int64_t nsys, const int64_t *nato
         return bail(bfail(PARM_E_DEVICE, "this library is built for gfx950 only"));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(bfail(PARM_E_DEVICE, "stream creation failed"));
+    // rebuild launch: as many systems staged in LDS as a workgroup's LDS
+    // holds next to the cell counters (all of them, up to 4096 atoms, on a
+    // 160 KB part), one workgroup per CU at a time
+    {
+        const size_t per = (size_t)(ndim + 1) * sizeof(double);
+        const size_t fixed = batch_rebuild_static_lds() + 1024;
+        const size_t lim = (size_t)prop.sharedMemPerBlock;
+        b->stage_max = lim > fixed ? (int)std::min<size_t>(4096, (lim - fixed) / per) : 0;
+        if (const char *e = std::getenv("PARM_BATCH_STAGE"))
+            b->stage_max = std::min(b->stage_max, std::max(0, std::atoi(e)));
+        b->rgrid = std::max(1, prop.multiProcessorCount) * (b->stage_max > 1024 ? 1 : 2);
+    }
     const size_t n = (size_t)tot;
     int rc;
     if ((rc = balloc((char **)&b->pos, n * b->pos_bytes)) ||
@@ -528,6 +546,7 @@ int parm_batch_create(int ndim, int precision, int64_t nsys, const int64_t *nato
         (rc = balloc(&b->keys, n)) || (rc = balloc(&b->cellidx, n)) ||
         (rc = balloc((char **)&b->xs, n * b->pos_bytes)) || (rc = balloc(&b->dsort, n)) ||
         (rc = balloc(&b->flags, (size_t)nsys)) || (rc = balloc(&b->which, (size_t)nsys)) ||
+        (rc = balloc(&b->work, (size_t)nsys + 1)) ||
         (rc = balloc(&b->gflag, 1)) || (rc = balloc(&b->npairs, (size_t)nsys)) ||
         (rc = balloc(&b->eatom, 4 * n)) || (rc = balloc(&b->sysout, 4 * (size_t)nsys)) ||
         (rc = balloc(&b->geom, (size_t)nsys)))
@@ -664,6 +683,7 @@ static int batch_pair(parm_batch *b, int kind, int ntypes, const double *eps,
     PB_CHECK(balloc(&b->tab, tab.size()));
     PB_HIP(hipMemcpy(b->tab, tab.data(), sizeof(PairParam) * tab.size(),
                      hipMemcpyHostToDevice));
+    b->tab0 = tab[0];
     PB_HIP(hipMemcpy(b->geom, b->geom_h.data(), sizeof(BatchGeom) * b->nsys,
                      hipMemcpyHostToDevice));
     PB_HIP(hipMemcpy(b->type, atype, sizeof(int32_t) * b->ntot, hipMemcpyHostToDevice));

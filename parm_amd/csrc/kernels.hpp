@@ -152,6 +152,7 @@ struct ListView {
     int tiled;
     int ntiles;            // tiles = workgroups of a tiled force launch
     int smax;              // largest staged stencil of any tile
+    int wcap;              // 32-bit words per lane the list array holds (cap / tpa)
     const int4 *tiles;
     const int2 *truns;     // kTileRuns runs per tile (tiles.hpp)
     const uint32_t *keys;  // sorted cell keys of the last rebuild
@@ -165,6 +166,8 @@ struct PairView {
     int single;            // 1: one type pair (parameters uniform)
     int harmonic;          // Repulsion with n == 2 for every type pair
     const PairParam *tab;
+    PairParam p0;          // tab[0] by value: a one-type kernel reads it from
+                           // its arguments (scalar registers), not from memory
 };
 
 // Integrator stage fused into the force kernel.
@@ -269,7 +272,12 @@ hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const 
                                 int32_t *ncount, int32_t *keys, int32_t *cellidx,
                                 int32_t *cstart, void *xs, double *dsort, uint32_t *flags,
                                 uint32_t *which, unsigned long long *npairs, uint32_t *gflag,
-                                hipStream_t s);
+                                int32_t *work, int grid, int stage_max, hipStream_t s);
+// LDS of a rebuild workgroup: systems of up to stage_max atoms keep their
+// sorted positions and diameters in dynamic LDS; the static part holds the
+// cell counters.  work: nsys + 1 ints (the worklist).
+size_t batch_rebuild_lds(int ndim, int stage_max);
+size_t batch_rebuild_static_lds();
 hipError_t launch_batch_sys_sums(int nsys, const BatchArrays &A, const double *eatom,
                                  double *out, hipStream_t s);
 constexpr int kBatchMaxCellsPerSystem = 4096;

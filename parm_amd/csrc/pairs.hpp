@@ -20,6 +20,29 @@ struct PairAcc {
     double contacts = 0.0, overlaps = 0.0;
 };
 
+// LJAttractRepulsePair's force factor fs (f = fs r) from d2 = |r|^2, with
+// rsix = (sigma / r)^6 and fmagR = 12 eps rsix (rsix - 1) (the energy and
+// virial terms) as by-products: one expression for every FP64 caller, so
+// they all round alike.
+__device__ __forceinline__ double lj64_fs(const PairParam &pp, double d2, double &rsix,
+                                          double &fmagR) {
+    const double idsq = rcp64(d2);
+    const double ir = pp.sig2 * idsq;  // sigma^2 / r^2
+    rsix = ir * ir * ir;               // sigma^6 / r^6
+    fmagR = pp.p6 * rsix * (rsix - 1.0);
+    return fmagR * idsq;
+}
+
+// The same in FP32 (mixed precision).
+__device__ __forceinline__ float lj32_fs(const PairParam &pp, float d2, float &rsix,
+                                         float &fmagR) {
+    const float idsq = __builtin_amdgcn_rcpf(d2);
+    const float ir = pp.sig2_f * idsq;
+    rsix = ir * ir * ir;
+    fmagR = pp.p6_f * rsix * (rsix - 1.0f);
+    return fmagR * idsq;
+}
+
 // FP64 LJAttractRepulsePair on one visit; returns false if outside cutoff.
 // Cutoff: the reference skips when d2 / sig^2 > cut^2; lim_lo / lim_hi bracket
 // that boundary so the division only runs within 1e-14 of it.
@@ -30,11 +53,8 @@ __device__ __forceinline__ void lj64(const PairParam &pp, const Vec<D> &r,
     bool in = d2 < pp.lim_lo ||
               (!(d2 > pp.lim_hi) && !(d2 / pp.sig2 > pp.cut2));
     if (!in) return;
-    const double idsq = rcp64(d2);
-    const double ir = pp.sig2 * idsq;         // sigma^2 / r^2
-    const double rsix = ir * ir * ir;         // sigma^6 / r^6
-    const double fmagR = pp.p6 * rsix * (rsix - 1.0);  // 12 eps s (s - 1)
-    const double fs = fmagR * idsq;
+    double rsix, fmagR;
+    const double fs = lj64_fs(pp, d2, rsix, fmagR);
 #pragma unroll
     for (int k = 0; k < D; k++) f[k] = fma(fs, r.c[k], f[k]);
     if (ENERGY) {
@@ -97,11 +117,8 @@ __device__ __forceinline__ void lj32f(const PairParam &pp, const float (&rf)[D],
         if (!(d2d < pp.lim_lo || (!(d2d > pp.lim_hi) && !(d2d / pp.sig2 > pp.cut2))))
             return;
     }
-    const float idsq = __builtin_amdgcn_rcpf(d2);
-    const float ir = pp.sig2_f * idsq;
-    const float rsix = ir * ir * ir;
-    const float fmagR = pp.p6_f * rsix * (rsix - 1.0f);
-    const float fs = fmagR * idsq;
+    float rsix, fmagR;
+    const float fs = lj32_fs(pp, d2, rsix, fmagR);
 #pragma unroll
     for (int k = 0; k < D; k++) f[k] = fmaf(fs, rf[k], f[k]);
     if (ENERGY) {

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomArrays A, ListView Lv,
         const PairParam *row = P.tab;
         PairParam p0;
         if (SINGLE)
-            p0 = P.tab[0];
+            p0 = P.p0;
         else
             row = P.tab + (size_t)A.type[i] * P.ntypes;
         int m = 0;

@@ -54,11 +54,8 @@ __device__ __forceinline__ void lj64x(const PairParam &pp, const Vec<D> &r,
         if (d2 > pp.lim_hi) return;
         if (exact_d2() / pp.sig2 > pp.cut2) return;
     }
-    const double idsq = rcp64(d2);
-    const double ir = pp.sig2 * idsq;
-    const double rsix = ir * ir * ir;
-    const double fmagR = pp.p6 * rsix * (rsix - 1.0);
-    const double fs = fmagR * idsq;
+    double rsix, fmagR;
+    const double fs = lj64_fs(pp, d2, rsix, fmagR);
 #pragma unroll
     for (int k = 0; k < D; k++) f[k] = fma(fs, r.c[k], f[k]);
     if (ENERGY) {
@@ -88,6 +85,12 @@ __device__ __forceinline__ Vec<D> offset_from(const Vec<D> &x, const Vec<D> &o, 
     }
     return r;
 }
+
+// List words per lane read ahead of the pair loop (two entries a word), and
+// the entries the one-type LJ loop evaluates side by side.
+constexpr int kChunkWords = 4;
+constexpr int kGroup = 4;
+static_assert(kChunkWords % (kGroup / 2) == 0, "groups tile a chunk");
 
 template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, int TPA>
 __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListView Lv,
@@ -125,6 +128,33 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     AtomIn<D> ain;
     const bool upd = STAGE != kStageSums && threadIdx.x < (unsigned)tcnt;
     if (upd) atom_load<D, STAGE>(A, a0 + (int)threadIdx.x, ain);
+    // this thread's atom and lane, and what its pair loop reads from memory
+    // (position, membership, entry count, the first chunk of list words),
+    // also loaded now: their latency hides behind the staging.  The first
+    // chunk is read whatever the lane's count (within the list array's
+    // wcap words per lane); words past the count are never used.
+    const int li = threadIdx.x / TPA, sub = threadIdx.x % TPA;
+    const int i = a0 + li;
+    const bool act = li < tcnt;
+    Vec<D> xi;
+#pragma unroll
+    for (int k = 0; k < D; k++) xi.c[k] = 0.0;
+    int rk = -1, nlane = 0;
+    const uint32_t *wp = Lv.nbr + ((size_t)i * TPA + sub);
+    const size_t wstride = (size_t)Lv.npad * TPA;
+    uint32_t cur[kChunkWords];
+#pragma unroll
+    for (int q = 0; q < kChunkWords; q++) cur[q] = 0u;
+    if (act) {
+        xi = PosStore<D>::load(pos[i]);
+        if (hdr.z >= 0) {
+            rk = A.rank[i];
+            nlane = Lv.lcount[(size_t)i * TPA + sub];
+#pragma unroll
+            for (int q = 0; q < kChunkWords; q++)
+                if (q < Lv.wcap) cur[q] = wp[(size_t)q * wstride];
+        }
+    }
     Vec<D> xo;
 #pragma unroll
     for (int k = 0; k < D; k++) xo.c[k] = 0.0;
@@ -200,9 +230,6 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     }
     const bool fold = __syncthreads_or(wide) != 0;
 
-    const int li = threadIdx.x / TPA, sub = threadIdx.x % TPA;
-    const int i = a0 + li;
-    const bool act = li < tcnt;
     PairAcc acc;
     double fi[D];
     float f32[D];
@@ -211,19 +238,12 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
         fi[k] = 0.0;
         f32[k] = 0.0f;
     }
-    Vec<D> xi;
-#pragma unroll
-    for (int k = 0; k < D; k++) xi.c[k] = 0.0;
-    if (act) xi = PosStore<D>::load(pos[i]);
-    const bool member = act && hdr.z >= 0 && A.rank[i] >= 0;
+    const bool member = act && hdr.z >= 0 && rk >= 0;
     if (member) {
-        const int nlane = Lv.lcount[(size_t)i * TPA + sub];
-        const uint32_t *wp = Lv.nbr + (size_t)i * TPA + sub;
-        const size_t wstride = (size_t)Lv.npad * TPA;
         const PairParam *prow = P.tab;
         PairParam p0;
         if (SINGLE)
-            p0 = P.tab[0];
+            p0 = P.p0;
         else
             prow = P.tab + (size_t)A.type[i] * P.ntypes;
         Vec<D> ri64 = xi;
@@ -317,17 +337,129 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                 }
             }
         };
-        // two lane entries per 32-bit word; the list streams from memory,
-        // so the word two ahead is loaded while the current one is evaluated
+        // kGroup entries (kGroup / 2 words) of the one-type LJ list at once,
+        // branch-free but for the in/out decision within a hair of the
+        // cutoff: their staged offsets read together, their force factors
+        // evaluated side by side, then added in list order (an entry outside
+        // the cutoff, or past the lane's count, adds 0 * r, which leaves the
+        // sum as it is; a slot past the count reads slot 0, which holds a
+        // staged offset whenever any entry exists).
+        auto group = [&](const uint32_t *wd, int e, auto fold_c) {
+            constexpr bool FOLD = decltype(fold_c)::value;
+            int sl[kGroup];
+            bool ok[kGroup], in[kGroup], band[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; g++) {
+                const uint32_t w = wd[g >> 1];
+                ok[g] = e + g < nlane;
+                sl[g] = ok[g] ? (int)((g & 1) ? (w >> 16) : (w & 0xffffu)) : 0;
+            }
+            if constexpr (PREC == kFP64) {
+                double r[kGroup][D], fs[kGroup];
+#pragma unroll
+                for (int g = 0; g < kGroup; g++) {
+#pragma unroll
+                    for (int k = 0; k < D; k++) {
+                        double d = ri64.c[k] - stc64[sl[g] * D + k];
+                        if (FOLD)
+                            d = d > box.halfL[k] ? d - box.L[k]
+                                                 : (d < -box.halfL[k] ? d + box.L[k] : d);
+                        r[g][k] = d;
+                    }
+                    double d2 = r[g][0] * r[g][0];
+#pragma unroll
+                    for (int k = 1; k < D; k++) d2 = fma(r[g][k], r[g][k], d2);
+                    const bool lo = d2 < p0.lim_lo;
+                    band[g] = ok[g] && !lo && !(d2 > p0.lim_hi);
+                    in[g] = ok[g] && lo;
+                    double rsix, fmagR;
+                    fs[g] = lj64_fs(p0, d2, rsix, fmagR);
+                }
+                bool anyband = false;
+#pragma unroll
+                for (int g = 0; g < kGroup; g++) anyband |= band[g];
+                if (anyband) {
+#pragma unroll
+                    for (int g = 0; g < kGroup; g++)
+                        if (band[g]) in[g] = !(exact_d2(sl[g]) / p0.sig2 > p0.cut2);
+                }
+#pragma unroll
+                for (int g = 0; g < kGroup; g++) {
+                    const double m = in[g] ? fs[g] : 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; k++) fi[k] = fma(m, r[g][k], fi[k]);
+                }
+            } else {
+                float r[kGroup][D], fs[kGroup];
+#pragma unroll
+                for (int g = 0; g < kGroup; g++) {
+#pragma unroll
+                    for (int k = 0; k < D; k++) {
+                        float d = ri[k] - stc32[sl[g] * D + k];
+                        if (FOLD) d = d > hLf[k] ? d - Lf[k] : (d < -hLf[k] ? d + Lf[k] : d);
+                        r[g][k] = d;
+                    }
+                    float d2 = r[g][0] * r[g][0];
+#pragma unroll
+                    for (int k = 1; k < D; k++) d2 = fmaf(r[g][k], r[g][k], d2);
+                    const bool lo = !(d2 > p0.lim_f * 0.99999f);
+                    band[g] = ok[g] && !lo && !(d2 > p0.lim_f * 1.00001f);
+                    in[g] = ok[g] && lo;
+                    float rsix, fmagR;
+                    fs[g] = lj32_fs(p0, d2, rsix, fmagR);
+                }
+                bool anyband = false;
+#pragma unroll
+                for (int g = 0; g < kGroup; g++) anyband |= band[g];
+                if (anyband) {
+#pragma unroll
+                    for (int g = 0; g < kGroup; g++) {
+                        if (band[g]) {
+                            const double d2d = exact_d2(sl[g]);
+                            in[g] = d2d < p0.lim_lo ||
+                                    (!(d2d > p0.lim_hi) && !(d2d / p0.sig2 > p0.cut2));
+                        }
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < kGroup; g++) {
+                    const float m = in[g] ? fs[g] : 0.0f;
+#pragma unroll
+                    for (int k = 0; k < D; k++) f32[k] = fmaf(m, r[g][k], f32[k]);
+                }
+            }
+        };
+        // two entries per 32-bit word; for the one-type LJ forms the lane's
+        // words stream from memory kChunkWords at a time, each chunk loaded
+        // while the one before it is evaluated (the first was loaded before
+        // the staging)
         auto loop = [&](auto fold_c) {
-            uint32_t w0 = nlane > 0 ? wp[0] : 0u;
-            uint32_t w1 = nlane > 2 ? wp[wstride] : 0u;
-            for (int m = 0; m < nlane; m += 2) {
-                const uint32_t cur = w0;
-                w0 = w1;
-                if (m + 4 < nlane) w1 = wp[(size_t)((m >> 1) + 2) * wstride];
-                pair((int)(cur & 0xffffu), fold_c);
-                if (m + 1 < nlane) pair((int)(cur >> 16), fold_c);
+            if constexpr (COMPACT && !ENERGY && KIND == kPairLJ) {
+                const int nw = (nlane + 1) >> 1;
+                for (int w0 = 0; w0 < nw; w0 += kChunkWords) {
+                    uint32_t nxt[kChunkWords];
+#pragma unroll
+                    for (int q = 0; q < kChunkWords; q++) {
+                        const int w = w0 + kChunkWords + q;
+                        nxt[q] = w < nw ? wp[(size_t)w * wstride] : 0u;
+                    }
+#pragma unroll
+                    for (int q = 0; q < kChunkWords; q += kGroup / 2)
+                        if (2 * (w0 + q) < nlane) group(cur + q, 2 * (w0 + q), fold_c);
+#pragma unroll
+                    for (int q = 0; q < kChunkWords; q++) cur[q] = nxt[q];
+                }
+            } else {
+                // the other forms one entry at a time, the word two ahead
+                // loaded while the current one is evaluated
+                uint32_t w0 = cur[0], w1 = cur[1];
+                for (int m = 0; m < nlane; m += 2) {
+                    const uint32_t c = w0;
+                    w0 = w1;
+                    if (m + 4 < nlane) w1 = wp[(size_t)((m >> 1) + 2) * wstride];
+                    pair((int)(c & 0xffffu), fold_c);
+                    if (m + 1 < nlane) pair((int)(c >> 16), fold_c);
+                }
             }
         };
         if (fold)

@@ -98,6 +98,7 @@ def test_nlcg_reset_descend_and_params():
     xr = ref.state()[0]
     assert np.abs(s.positions() - xr).max() <= 1e-12 * np.abs(xr).max()
     c.set_params(2e-3, 0.5, 5.0)
+    c.reset()
     cr_info = c.info()
     assert cr_info["k"] == 0 and cr_info["vl"] == cr_info["al"] == cr_info["fl"]
     c.close()

@@ -6,9 +6,10 @@
   NListed<IEpsISigExpAtom, RepulsionPair>        parm_pair_repulsion_indexed (RepulsionII)
 
 Bars as for the first two kinds: pair set bit-identical, FP64 forces within
-1e-12 of sum_j |f_ij| per atom, U / W / S within 1e-12 relative, contacts
+1e-12 of sum_j |f_ij| per atom plus 1e-13 of the scale of the force's terms
+(which cancel near the LJ minimum), U / W / S within 1e-12 relative, contacts
 exact, 10-step Verlet trajectories within 1e-12 relative, mixed precision
-forces within 2e-5 normalised (+ 1e-6 absolute for repulsion).
+forces within 2e-5 normalised plus 1e-6 of the terms' scale.
 """
 import numpy as np
 import pytest
@@ -89,8 +90,11 @@ def test_pair_kind_forces_energy_and_pairs(kind, ndim, n, rho, skin):
     assert np.array_equal(f_dev, first) and np.array_equal(l_dev, last)
     s.set_forces(False)
     ref = O.forces(x, L, table, first, last)
-    err = S.normalised_force_error(s.forces(), ref["f"], ref["fabs"])
-    assert err <= 1e-12, err
+    # FP64 rounds each term of a pair force, not their (cancelling) sum: the
+    # bar adds 1e-13 of the terms' scale to 1e-12 of sum_j |f_ij|.
+    d = np.sqrt(((s.forces() - ref["f"]) ** 2).sum(1))
+    bar = 1e-12 * ref["fabs"] + 1e-13 * _term_scale(kind, x, L, table, first, last)
+    assert np.all(d <= bar), float(np.max(d / np.maximum(bar, 1e-300)))
     sums = s.pair_sums()
     assert abs(sums["U"] - ref["U"]) <= 1e-12 * abs(ref["U"]) + 1e-13
     assert abs(sums["W"] - ref["W"]) <= 1e-12 * abs(ref["W"]) + 1e-13
