@@ -217,6 +217,18 @@ int parm_add_pair_forces(parm_system *sys);
 int parm_pair_sums(parm_system *sys, double *U, double *W, double *S,
                    uint64_t *contacts, uint64_t *overlaps);
 
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The sums of parm_pair_sums, queued on the system's stream and
+ * written to out, a DEVICE pointer to 13 doubles: U, W, S[9] (row-major
+ * ndim x ndim in the first ndim*ndim), contacts, overlaps.  It neither
+ * allocates nor synchronises (read out after parm_synchronize).  Its
+ * workgroup partials live in a fixed device array of the library (about
+ * 8 MiB per GPU, held until the process ends), so systems of at most
+ * 2^21 atoms (larger: PARM_E_UNSUPPORTED), and calls that overlap on one GPU
+ * race (order them on one stream, or synchronise between them). */
+int parm_pair_sums_noalloc(parm_system *sys, double *out);
+
 /* AtomGroup::kinetic_energy(originv) (box.cpp:401-411), momentum()
  * (box.cpp:251-260) and mass() (box.cpp:239-247).  originv may be NULL (0). */
 int parm_kinetic_energy(parm_system *sys, const double *originv, double *K);

@@ -38,6 +38,7 @@ EXPORTS = (
     "parm_nlcg_reset", "parm_nlcg_set_forces", "parm_nlcg_timestep", "parm_nlcg_descend",
     "parm_nlcg_observables", "parm_nlcg_info",
     "parm_snapshot_begin", "parm_snapshot_read", "parm_snapshot_write_xyz",
+    "parm_pair_sums_noalloc",
     "parm_batch_create", "parm_batch_destroy", "parm_batch_info",
     "parm_batch_set_masses", "parm_batch_set_state", "parm_batch_get_state",
     "parm_batch_pair_repulsion", "parm_batch_pair_lj_attract_repulse",
@@ -131,6 +132,7 @@ def lib():
             "parm_nlcg_observables": (i32, [_vp, _dp, _dp, _dp]),
             "parm_nlcg_info": (i32, [_vp, _dp]),
             "parm_snapshot_begin": (i32, [_vp, i32, C.POINTER(C.c_int64)]),
+            "parm_pair_sums_noalloc": (i32, [_vp, _vp]),
             "parm_snapshot_read": (i32, [_vp, i64, _dp, _dp, _dp]),
             "parm_snapshot_write_xyz": (i32, [_vp, i64, C.c_char_p, i32, C.c_char_p,
                                               C.c_char_p, i32]),

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <iterator>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1449,6 +1450,83 @@ int parm_pair_sums(parm_system *s, double *U, double *W, double *S, uint64_t *co
         for (int k = 0; k < s->ndim * s->ndim; k++) S[k] = o[2 + k];
     if (contacts) *contacts = (uint64_t)std::llround(o[11] / 2);
     if (overlaps) *overlaps = (uint64_t)std::llround(o[12] / 2);
+    return PARM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------- sums without allocation
+// Scratch of parm_pair_sums_noalloc: the per-workgroup partials of one force
+// launch (kRedWidth fields x force_blocks(n, 8) workgroups) and the reduced
+// fields, for systems of at most kNoAllocMaxAtoms atoms.  Each GPU holds its
+// own copy for the life of the process; host code reaches it only through
+// the per-device address from hipGetSymbolAddress.
+constexpr int kNoAllocMaxAtoms = 1 << 21;
+constexpr size_t kNoAllocParts = (size_t)kNoAllocMaxAtoms * 8 / kBlock;  // workgroups
+static __device__ __align__(256) double scratch[kRedWidth * kNoAllocParts + kRedWidth];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The calling thread's current device's scratch address (looked up once per
+// device, with that device current).
+int scratch_address(int device, double **out) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(PARM_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PARM_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                           hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return PARM_OK;
+}
+
+// Reduced fields -> the caller's layout: U, W, S[9] (ndim x ndim used),
+// contacts, overlaps (each pair counted once).
+// (red == nullptr: all zero, for a system with no atoms).
+__global__ void k_sums_out(const double *__restrict__ red, double *__restrict__ out) {
+    const int k = threadIdx.x;
+    if (k >= 13) return;
+    const double v = red ? red[k] : 0.0;
+    out[k] = k < 11 ? v : v / 2;
+}
+}  // namespace
+
+extern "C" {
+
+int parm_pair_sums_noalloc(parm_system *s, double *out) {
+    PARM_CHECK(check_sys(s));
+    if (!out) return fail(PARM_E_INVALID, "out is null");
+    if (s->n > kNoAllocMaxAtoms)
+        return fail(PARM_E_UNSUPPORTED,
+                    "parm_pair_sums_noalloc: at most " + std::to_string(kNoAllocMaxAtoms) +
+                        " atoms (its scratch is a fixed device array)");
+    PARM_CHECK(require_ready(s));
+    PARM_CHECK(set_device(s));
+    double *scr = nullptr;
+    PARM_CHECK(scratch_address(s->device, &scr));
+    ReduceBuf R = s->redbuf(kSlotScratch);
+    R.part = scr;
+    R.pstride = force_blocks(std::max(s->n, 1), 8);  // <= kNoAllocParts
+    R.out = scr + kRedWidth * kNoAllocParts;
+    if (s->n > 0) {
+        StepCoef c{};
+        PARM_HIP(launch_force(s->ndim, s->prec, kStageSums, true, s->arrays(), s->listview(),
+                              s->pairview(), s->box, c, R, s->stream));
+        hipLaunchKernelGGL(k_sums_out, dim3(1), dim3(64), 0, s->stream, R.out, out);
+    } else {
+        // an empty system has no pairs: zero sums, by a kernel (no memset)
+        hipLaunchKernelGGL(k_sums_out, dim3(1), dim3(64), 0, s->stream,
+                           (const double *)nullptr, out);
+    }
+    PARM_HIP(hipGetLastError());
     return PARM_OK;
 }
 
