@@ -36,6 +36,15 @@
  *       eps_tab : eps per type; sig_tab : sigma per type; p3_tab unused
  *   OR_REPULSION_II        NListed<IEpsISigExpAtom, RepulsionPair>
  *       eps_tab, sig_tab : ntypes x ntypes rows; p3_tab : exponent per type
+ *
+ * What pins each part (tests/test_oracle.py, tests/test_oracle_kinds.py):
+ * exact rationals, closed forms, 50-digit gradients, brute-force lists and
+ * the reference's own test assertions; for CollectionNLCG, equilibrium at
+ * the goal pressure, the exact reset/descend steps and finite termination
+ * of conjugate gradients on an exact quadratic.  Parity unpinned: the
+ * Polak-Ribiere numerator's -f_new.f_old term of CollectionNLCG's beta
+ * (it vanishes under exact line searches on a quadratic, and the
+ * convergence tests would pass without it).
  */
 #ifndef PARM_ORACLE_H
 #define PARM_ORACLE_H

@@ -10,9 +10,10 @@
 //
 // Per step:  k_batch_force    forces + kick (+ next drift, |x - xref| per atom)
 //            k_batch_worklist the systems whose skin flag is set, as a list
-//            k_batch_rebuild  a fixed grid taking the listed systems in turn:
-//                             cell sort in LDS, sorted positions staged in
-//                             LDS, exact list build, xref = x
+//            k_batch_sort     a fixed grid taking the listed systems in turn:
+//                             cell sort in LDS, sorted copies, xref = x
+//            k_batch_build    a fixed grid over slices of the listed
+//                             systems: the exact list build, a thread per atom
 //            k_batch_skin     one wavefront per system: top-2 of |x - xref|
 //                             -> the system's flag for the next rebuild
 #include "common.hpp"
@@ -22,6 +23,8 @@
 namespace parm {
 
 // ---------------------------------------------------------------- forces
+constexpr int kBatchAhead = 4;  // list entries per lane evaluated before the loop
+
 template <int D, int KIND, int PREC, bool SINGLE, bool HARM, int STAGE, bool ENERGY,
           int TPA>
 __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView Lv,
@@ -32,7 +35,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
         (STAGE == kStageVerletKickDrift || STAGE == kStageDampedKickDrift);
     constexpr bool DAMPED =
         (STAGE == kStageDampedKick || STAGE == kStageDampedKickDrift);
-    const int gt = blockIdx.x * kBlock + threadIdx.x;
+    // consecutive workgroups (consecutive atoms: one system's, mostly) on one
+    // XCD, so each system's positions, which its atoms' neighbour gathers
+    // read, are held in one L2 rather than in all eight
+    const int gt = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const int i = gt / TPA;
     const int sub = gt % TPA;
     const bool act = i < A.ntot;
@@ -69,9 +75,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
         }
     }
     if (act && Lv.built) {
-        const int cnt = Lv.ncount[i];
         const size_t stride = (size_t)Lv.npad * TPA;
         const uint32_t *nb = Lv.nbr + (size_t)i * TPA + sub;
+        // the lane's first kBatchAhead entries: their list words are read
+        // together with the entry count (any word within the list array's
+        // wcap per lane is in bounds; those past the count are not used),
+        // then their neighbours' positions are gathered together
+        uint32_t w[kBatchAhead];
+#pragma unroll
+        for (int k = 0; k < kBatchAhead; k++)
+            w[k] = k < Lv.wcap ? nb[(size_t)k * stride] : 0u;
+        const int cnt = Lv.ncount[i];
         const int tb = Lv.type_bits;
         const uint32_t tmask = (1u << tb) - 1u;
         const PairParam *row = P.tab;
@@ -80,11 +94,24 @@ __global__ __launch_bounds__(kBlock) void k_batch_force(BatchArrays A, ListView 
             p0 = P.p0;
         else
             row = P.tab + (size_t)A.type[i] * P.ntypes;
-        int q = sub;
-        size_t off = 0;
-        // the next two entries are loaded while the current two are evaluated
-        uint32_t n0 = q < cnt ? nb[0] : 0u;
-        uint32_t n1 = q + TPA < cnt ? nb[stride] : 0u;
+        PT pw[kBatchAhead];
+#pragma unroll
+        for (int k = 0; k < kBatchAhead; k++)
+            if (sub + k * TPA < cnt) pw[k] = pos[w[k] >> tb];
+#pragma unroll
+        for (int k = 0; k < kBatchAhead; k++) {
+            if (sub + k * TPA < cnt) {
+                const Vec<D> r = diff<D>(xi, PosStore<D>::load(pw[k]), box);
+                visit<D, KIND, PREC, ENERGY, HARM>(SINGLE ? p0 : row[w[k] & tmask], r, fi,
+                                                   f32, acc);
+            }
+        }
+        // the rest: the next two entries loaded while the current two are
+        // evaluated
+        int q = sub + kBatchAhead * TPA;
+        size_t off = (size_t)kBatchAhead * stride;
+        uint32_t n0 = q < cnt ? nb[off] : 0u;
+        uint32_t n1 = q + TPA < cnt ? nb[off + stride] : 0u;
         for (; q + TPA < cnt; q += 2 * TPA, off += 2 * stride) {
             const uint32_t e0 = n0;
             const uint32_t e1 = n1;
@@ -236,70 +263,94 @@ __global__ __launch_bounds__(kBlock) void k_batch_skin(int nsys, BatchArrays A,
 // -------------------------------------------------------------- rebuild
 // The systems to rebuild at the end of a step are those whose skin flag is
 // set (or all, when forced).  k_batch_worklist turns the flags into a list
-// (system ids in ascending order, count first); k_batch_rebuild then runs a
-// fixed grid of workgroups that take the listed systems in turn, so the
-// launch shape does not depend on how many systems rebuild (the step stays
-// graph-capturable) and no workgroup is started for a system that keeps its
-// list.  Per system, one workgroup of kRebuildThreads: cell keys and a
-// stable counting sort of the system's atoms by cell in LDS; the sorted
-// positions and diameters are staged in LDS (systems of up to `stage_max`
-// atoms; larger ones are copied to xs / dsort in memory) so each candidate
-// is an LDS read; then the exact list build over the periodic 27-cell
-// stencil (candidates in cell order, then sorted order: deterministic), four
-// candidates in flight per thread; then xref = x.  With `alt` (the
-// positions one step ahead), |alt - xref| is recomputed for the next skin
-// test.
+// (system ids in ascending order, count first) and counts each listed
+// system's slices of kBuildSlice atoms (exclusive prefix in wslice).  Then
+// two launches on fixed grids, so the launch shapes do not depend on how
+// many systems rebuild (the step stays graph-capturable) and no workgroup is
+// started for a system that keeps its list:
+//   k_batch_sort   one workgroup of kSortThreads per listed system: cell
+//                  keys, a stable counting sort of the system's atoms by
+//                  cell in LDS, the cell starts, sorted copies of the
+//                  positions and diameters, xref = x;
+//   k_batch_build  one workgroup of kBuildSlice threads per slice of a
+//                  listed system, a thread per atom: the exact list build
+//                  over the periodic 27-cell stencil (candidates in cell
+//                  order, then sorted order: deterministic), four candidates
+//                  in flight per thread.  A large system's build is spread
+//                  over many workgroups.
+// With `alt` (the positions one step ahead), |alt - xref| is recomputed for
+// the next skin test.
 constexpr int kBatchMaxCells = kBatchMaxCellsPerSystem;
-constexpr int kRebuildThreads = 1024;
+constexpr int kSortThreads = 1024;
 constexpr int kWorklistThreads = 1024;
+constexpr int kBuildSlice = 256;
 
 __global__ __launch_bounds__(kWorklistThreads) void k_batch_worklist(
-    int nsys, const uint32_t *__restrict__ flags, int force, int32_t *__restrict__ work) {
-    __shared__ int wsum[kWorklistThreads / kWave];
+    int nsys, const uint32_t *__restrict__ flags, int force,
+    const BatchGeom *__restrict__ geom, int32_t *__restrict__ work,
+    int32_t *__restrict__ wslice) {
+    __shared__ int wsum[kWorklistThreads / kWave][2];
     const int t = threadIdx.x;
     const int per = (nsys + kWorklistThreads - 1) / kWorklistThreads;
     const int s0 = t * per;
-    int loc = 0;
+    int loc = 0, lsl = 0;
     for (int q = 0; q < per; q++) {
         const int s = s0 + q;
-        loc += (s < nsys && (force || flags[s] != 0u)) ? 1 : 0;
+        if (s < nsys && (force || flags[s] != 0u)) {
+            loc++;
+            lsl += (geom[s].n + kBuildSlice - 1) / kBuildSlice;
+        }
     }
     const int lane = t % kWave, w = t / kWave;
-    int incl = loc;
+    int incl = loc, isl = lsl;
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
-        const int y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
+        const int y = __shfl_up(incl, o), z = __shfl_up(isl, o);
+        if (lane >= o) {
+            incl += y;
+            isl += z;
+        }
     }
-    if (lane == kWave - 1) wsum[w] = incl;
+    if (lane == kWave - 1) {
+        wsum[w][0] = incl;
+        wsum[w][1] = isl;
+    }
     __syncthreads();
-    int pos = incl - loc, tot = 0;
+    int pos = incl - loc, psl = isl - lsl, tot = 0, totsl = 0;
     for (int q = 0; q < kWorklistThreads / kWave; q++) {
-        if (q < w) pos += wsum[q];
-        tot += wsum[q];
+        if (q < w) {
+            pos += wsum[q][0];
+            psl += wsum[q][1];
+        }
+        tot += wsum[q][0];
+        totsl += wsum[q][1];
     }
     for (int q = 0; q < per; q++) {
         const int s = s0 + q;
-        if (s < nsys && (force || flags[s] != 0u)) work[1 + pos++] = s;
+        if (s < nsys && (force || flags[s] != 0u)) {
+            work[1 + pos] = s;
+            wslice[pos] = psl;
+            pos++;
+            psl += (geom[s].n + kBuildSlice - 1) / kBuildSlice;
+        }
     }
-    if (t == 0) work[0] = tot;
+    if (t == 0) {
+        work[0] = tot;
+        wslice[tot] = totsl;
+    }
 }
 
-template <int D, bool STAGED>
-__device__ __forceinline__ void rebuild_system(
+template <int D>
+__device__ __forceinline__ void sort_system(
     const int s, const BatchArrays &A, const typename PosStore<D>::T *__restrict__ pos,
-    const typename PosStore<D>::T *__restrict__ alt, int count_rebuild, double skin,
-    int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
-    int32_t *__restrict__ ncount, int32_t *__restrict__ keys, int32_t *__restrict__ cellidx,
+    int count_rebuild, int32_t *__restrict__ keys, int32_t *__restrict__ cellidx,
     int32_t *__restrict__ cstart, typename PosStore<D>::T *__restrict__ xs,
     double *__restrict__ dsort, uint32_t *__restrict__ flags, uint32_t *__restrict__ which,
-    unsigned long long *__restrict__ npairs, uint32_t *__restrict__ gflag, int *hist,
-    unsigned long long *red, double *lx, double *ld, int stage_max) {
+    unsigned long long *__restrict__ nent, int *hist) {
     using PT = typename PosStore<D>::T;
-    constexpr int NT = kRebuildThreads;
+    constexpr int NT = kSortThreads;
     const BatchGeom G = A.geom[s];
-    const int off = G.off, n = G.n, nc0 = G.grid.nc[0], nc1 = G.grid.nc[1],
-              nc2 = G.grid.nc[2], ncell = G.grid.ncell;
+    const int off = G.off, n = G.n, ncell = G.grid.ncell;
     const int t = threadIdx.x;
     for (int c = t; c < ncell; c += NT) hist[c] = 0;
     __syncthreads();
@@ -384,129 +435,16 @@ __device__ __forceinline__ void rebuild_system(
             }
         }
     }
-    // hist[c] is now the end of cell c (system-local sorted index).  Sorted
-    // copies of the positions and diameters (in LDS when staged), and
-    // xref = x (the reference stores lastlocs inside the rebuild loop).
+    // sorted copies of the positions and diameters, and xref = x (the
+    // reference stores lastlocs inside the rebuild loop)
     for (int u = t; u < n; u += NT) {
         const int i = cellidx[off + u];
-        const PT p = pos[i];
-        if constexpr (STAGED) {
-            const Vec<D> x = PosStore<D>::load(p);
-#pragma unroll
-            for (int k = 0; k < D; k++) lx[k * stage_max + u] = x.c[k];
-            ld[u] = A.diam[i];
-        } else {
-            xs[off + u] = p;
-            dsort[off + u] = A.diam[i];
-        }
+        xs[off + u] = pos[i];
+        dsort[off + u] = A.diam[i];
     }
     for (int k = t; k < n; k += NT) static_cast<PT *>(A.xref_w)[off + k] = pos[off + k];
-    __threadfence_block();
-    __syncthreads();
-    auto cand_x = [&](int v) {
-        if constexpr (STAGED) {
-            Vec<D> x;
-#pragma unroll
-            for (int k = 0; k < D; k++) x.c[k] = lx[k * stage_max + v];
-            return x;
-        } else {
-            return PosStore<D>::load(xs[off + v]);
-        }
-    };
-    auto cand_d = [&](int v) {
-        if constexpr (STAGED)
-            return ld[v];
-        else
-            return dsort[off + v];
-    };
-    // list build: thread per atom in sorted order
-    unsigned long long mycnt = 0;
-    const int lo0 = nc0 >= 3 ? -1 : 0, lo1 = (D >= 2 && nc1 >= 3) ? -1 : 0,
-              lo2 = (D == 3 && nc2 >= 3) ? -1 : 0;
-    for (int u = t; u < n; u += NT) {
-        const int i = cellidx[off + u];
-        const Vec<D> xi = cand_x(u);
-        const double di = cand_d(u);
-        int key = keys[i];
-        int ci[3] = {0, 0, 0};
-        ci[0] = key % nc0;
-        key /= nc0;
-        if (D >= 2) {
-            ci[1] = key % nc1;
-            key /= nc1;
-        }
-        if (D == 3) ci[2] = key % nc2;
-        int cnt = 0;
-        for (int oz = lo2; oz <= -lo2; oz++)
-            for (int oy = lo1; oy <= -lo1; oy++)
-                for (int ox = lo0; ox <= -lo0; ox++) {
-                    int q0 = ci[0] + ox, q1 = ci[1] + oy, q2 = ci[2] + oz;
-                    q0 = q0 < 0 ? q0 + nc0 : (q0 >= nc0 ? q0 - nc0 : q0);
-                    q1 = q1 < 0 ? q1 + nc1 : (q1 >= nc1 ? q1 - nc1 : q1);
-                    q2 = q2 < 0 ? q2 + nc2 : (q2 >= nc2 ? q2 - nc2 : q2);
-                    const int cc = (D == 3 ? (q2 * nc1 + q1) * nc0 : q1 * nc0) + q0;
-                    const int s0 = cc > 0 ? hist[cc - 1] : 0, s1 = hist[cc];
-                    for (int v0 = s0; v0 < s1; v0 += 4) {
-                        // four candidates' loads in flight together
-                        Vec<D> xj[4];
-                        double dj[4];
-#pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            const int v = v0 + q < s1 ? v0 + q : s0;
-                            xj[q] = cand_x(v);
-                            dj[q] = cand_d(v);
-                        }
-#pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            const int v = v0 + q;
-                            if (v >= s1 || v == u) continue;
-                            // minimum image by rint: equal to the reference's
-                            // remainder up to rounding far inside the 1e-6
-                            // band, where the reference's predicate decides
-                            double d2 = 0.0;
-#pragma unroll
-                            for (int k = 0; k < D; k++) {
-                                double dk = xi.c[k] - xj[q].c[k];
-                                dk -= G.box.L[k] * rint(dk * G.box.invL[k]);
-                                d2 = fma(dk, dk, d2);
-                            }
-                            const double cut = (di + dj[q]) / 2 + skin;
-                            const double c2 = cut * cut;
-                            const bool pass =
-                                d2 < c2 * (1.0 - 1e-6) ||
-                                (d2 <= c2 * (1.0 + 1e-6) &&
-                                 in_list<D>(xi, xj[q], di, dj[q], G.box, skin));
-                            if (pass) {
-                                if (cnt < cap) {
-                                    const int j = cellidx[off + v];
-                                    nbr[untiled_entry(cnt, i, tpa, npad)] =
-                                        ((uint32_t)j << type_bits) | (uint32_t)A.type[j];
-                                }
-                                cnt++;
-                            }
-                        }
-                    }
-                }
-        ncount[i] = cnt < cap ? cnt : cap;
-        if (cnt > cap) atomicOr(gflag, kFlagOverflow);
-        mycnt += (unsigned long long)cnt;
-        if (alt) {
-            const Vec<D> xa = PosStore<D>::load(alt[i]);
-            Vec<D> dd;
-#pragma unroll
-            for (int d = 0; d < D; d++) dd.c[d] = xa.c[d] - xi.c[d];
-            A.disp[i] = sqrt(sqnorm<D>(dd));
-        }
-    }
-    unsigned long long sum = mycnt;
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-    if (t % kWave == 0) red[t / kWave] = sum;
-    __syncthreads();
     if (t == 0) {
-        unsigned long long tot = 0;
-        for (int q = 0; q < NT / kWave; q++) tot += red[q];
-        npairs[s] = tot / 2;
+        nent[s] = 0ull;
         if (count_rebuild) which[s] += 1u;
         flags[s] = 0u;
     }
@@ -515,32 +453,138 @@ __device__ __forceinline__ void rebuild_system(
 }
 
 template <int D>
-__global__ __launch_bounds__(kRebuildThreads) void k_batch_rebuild(
-    BatchArrays A, const typename PosStore<D>::T *__restrict__ pos,
-    const typename PosStore<D>::T *__restrict__ alt, int count_rebuild, double skin,
-    int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
-    int32_t *__restrict__ ncount, int32_t *__restrict__ keys, int32_t *__restrict__ cellidx,
-    int32_t *__restrict__ cstart, typename PosStore<D>::T *__restrict__ xs,
-    double *__restrict__ dsort, uint32_t *__restrict__ flags, uint32_t *__restrict__ which,
-    unsigned long long *__restrict__ npairs, uint32_t *__restrict__ gflag,
-    const int32_t *__restrict__ work, int stage_max) {
+__global__ __launch_bounds__(kSortThreads) void k_batch_sort(
+    BatchArrays A, const typename PosStore<D>::T *__restrict__ pos, int count_rebuild,
+    int32_t *__restrict__ keys, int32_t *__restrict__ cellidx, int32_t *__restrict__ cstart,
+    typename PosStore<D>::T *__restrict__ xs, double *__restrict__ dsort,
+    uint32_t *__restrict__ flags, uint32_t *__restrict__ which,
+    unsigned long long *__restrict__ nent, const int32_t *__restrict__ work) {
     __shared__ int hist[kBatchMaxCells];
-    __shared__ unsigned long long red[kRebuildThreads / kWave];
-    extern __shared__ double lstage[];
-    double *lx = lstage;                   // D x stage_max positions (SoA)
-    double *ld = lstage + D * stage_max;   // stage_max diameters
     const int count = work[0];
-    for (int q = blockIdx.x; q < count; q += gridDim.x) {
-        const int s = work[1 + q];
-        if (A.geom[s].n <= stage_max)
-            rebuild_system<D, true>(s, A, pos, alt, count_rebuild, skin, type_bits, tpa, cap,
-                                    npad, nbr, ncount, keys, cellidx, cstart, xs, dsort, flags,
-                                    which, npairs, gflag, hist, red, lx, ld, stage_max);
-        else
-            rebuild_system<D, false>(s, A, pos, alt, count_rebuild, skin, type_bits, tpa, cap,
-                                     npad, nbr, ncount, keys, cellidx, cstart, xs, dsort,
-                                     flags, which, npairs, gflag, hist, red, lx, ld,
-                                     stage_max);
+    for (int q = blockIdx.x; q < count; q += gridDim.x)
+        sort_system<D>(work[1 + q], A, pos, count_rebuild, keys, cellidx, cstart, xs, dsort,
+                       flags, which, nent, hist);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBuildSlice) void k_batch_build(
+    BatchArrays A, const typename PosStore<D>::T *__restrict__ alt, double skin,
+    int type_bits, int tpa, int cap, int npad, uint32_t *__restrict__ nbr,
+    int32_t *__restrict__ ncount, const int32_t *__restrict__ keys,
+    const int32_t *__restrict__ cellidx, const int32_t *__restrict__ cstart,
+    const typename PosStore<D>::T *__restrict__ xs, const double *__restrict__ dsort,
+    unsigned long long *__restrict__ nent, uint32_t *__restrict__ gflag,
+    const int32_t *__restrict__ work, const int32_t *__restrict__ wslice) {
+    __shared__ unsigned long long red[kBuildSlice / kWave];
+    const int count = work[0];
+    const int total = wslice[count];
+    const int t = threadIdx.x;
+    for (int gs = blockIdx.x; gs < total; gs += gridDim.x) {
+        // the listed system holding slice gs: the last q with wslice[q] <= gs
+        int lo = 0, hi = count;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (wslice[mid] <= gs)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const int s = work[1 + lo];
+        const BatchGeom G = A.geom[s];
+        const int off = G.off, n = G.n, nc0 = G.grid.nc[0], nc1 = G.grid.nc[1],
+                  nc2 = G.grid.nc[2];
+        const int u = (gs - wslice[lo]) * kBuildSlice + t;
+        unsigned long long mycnt = 0;
+        if (u < n) {
+            const int lo0 = nc0 >= 3 ? -1 : 0, lo1 = (D >= 2 && nc1 >= 3) ? -1 : 0,
+                      lo2 = (D == 3 && nc2 >= 3) ? -1 : 0;
+            const int i = cellidx[off + u];
+            const Vec<D> xi = PosStore<D>::load(xs[off + u]);
+            const double di = dsort[off + u];
+            int key = keys[i];
+            int ci[3] = {0, 0, 0};
+            ci[0] = key % nc0;
+            key /= nc0;
+            if (D >= 2) {
+                ci[1] = key % nc1;
+                key /= nc1;
+            }
+            if (D == 3) ci[2] = key % nc2;
+            int cnt = 0;
+            for (int oz = lo2; oz <= -lo2; oz++)
+                for (int oy = lo1; oy <= -lo1; oy++)
+                    for (int ox = lo0; ox <= -lo0; ox++) {
+                        int q0 = ci[0] + ox, q1 = ci[1] + oy, q2 = ci[2] + oz;
+                        q0 = q0 < 0 ? q0 + nc0 : (q0 >= nc0 ? q0 - nc0 : q0);
+                        q1 = q1 < 0 ? q1 + nc1 : (q1 >= nc1 ? q1 - nc1 : q1);
+                        q2 = q2 < 0 ? q2 + nc2 : (q2 >= nc2 ? q2 - nc2 : q2);
+                        const int cc = (D == 3 ? (q2 * nc1 + q1) * nc0 : q1 * nc0) + q0;
+                        const int s0 = cstart[G.cell_off + cc] - off;
+                        const int s1 = cstart[G.cell_off + cc + 1] - off;
+                        for (int v0 = s0; v0 < s1; v0 += 4) {
+                            // four candidates' loads in flight together
+                            Vec<D> xj[4];
+                            double dj[4];
+#pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                const int v = v0 + q < s1 ? v0 + q : s0;
+                                xj[q] = PosStore<D>::load(xs[off + v]);
+                                dj[q] = dsort[off + v];
+                            }
+#pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                const int v = v0 + q;
+                                if (v >= s1 || v == u) continue;
+                                // minimum image by rint: equal to the
+                                // reference's remainder up to rounding far
+                                // inside the 1e-6 band, where the reference's
+                                // predicate decides
+                                double d2 = 0.0;
+#pragma unroll
+                                for (int k = 0; k < D; k++) {
+                                    double dk = xi.c[k] - xj[q].c[k];
+                                    dk -= G.box.L[k] * rint(dk * G.box.invL[k]);
+                                    d2 = fma(dk, dk, d2);
+                                }
+                                const double cut = (di + dj[q]) / 2 + skin;
+                                const double c2 = cut * cut;
+                                const bool pass =
+                                    d2 < c2 * (1.0 - 1e-6) ||
+                                    (d2 <= c2 * (1.0 + 1e-6) &&
+                                     in_list<D>(xi, xj[q], di, dj[q], G.box, skin));
+                                if (pass) {
+                                    if (cnt < cap) {
+                                        const int j = cellidx[off + v];
+                                        nbr[untiled_entry(cnt, i, tpa, npad)] =
+                                            ((uint32_t)j << type_bits) | (uint32_t)A.type[j];
+                                    }
+                                    cnt++;
+                                }
+                            }
+                        }
+                    }
+            ncount[i] = cnt < cap ? cnt : cap;
+            if (cnt > cap) atomicOr(gflag, kFlagOverflow);
+            mycnt = (unsigned long long)cnt;
+            if (alt) {
+                const Vec<D> xa = PosStore<D>::load(alt[i]);
+                Vec<D> dd;
+#pragma unroll
+                for (int d = 0; d < D; d++) dd.c[d] = xa.c[d] - xi.c[d];
+                A.disp[i] = sqrt(sqnorm<D>(dd));
+            }
+        }
+        unsigned long long sum = mycnt;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (t % kWave == 0) red[t / kWave] = sum;
+        __syncthreads();
+        if (t == 0) {
+            unsigned long long tot = 0;
+            for (int q = 0; q < kBuildSlice / kWave; q++) tot += red[q];
+            atomicAdd(&nent[s], tot);
+        }
+        __syncthreads();
     }
 }
 
@@ -716,38 +760,35 @@ hipError_t launch_batch_skin(int nsys, const BatchArrays &A, double skin, uint32
     return hipGetLastError();
 }
 
-size_t batch_rebuild_lds(int ndim, int stage_max) {
-    return (size_t)(ndim + 1) * (size_t)stage_max * sizeof(double);
-}
-
-size_t batch_rebuild_static_lds() {
-    return sizeof(int) * kBatchMaxCells + sizeof(unsigned long long) * (kRebuildThreads / kWave) +
-           sizeof(int) * (kRebuildThreads / kWave);
-}
-
 hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const void *pos,
                                 const void *alt, int force, int count_rebuild, double skin,
                                 int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                                 int32_t *ncount, int32_t *keys, int32_t *cellidx,
                                 int32_t *cstart, void *xs, double *dsort, uint32_t *flags,
-                                uint32_t *which, unsigned long long *npairs, uint32_t *gflag,
-                                int32_t *work, int grid, int stage_max, hipStream_t s) {
+                                uint32_t *which, unsigned long long *nent, uint32_t *gflag,
+                                int32_t *work, int32_t *wslice, int sort_grid, int build_grid,
+                                hipStream_t s) {
     if (nsys <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_batch_worklist, dim3(1), dim3(kWorklistThreads), 0, s, nsys, flags,
-                       force, work);
-    const int g = grid < nsys ? grid : nsys;
-    // systems of up to stage_max atoms build from LDS
-    const size_t lds = batch_rebuild_lds(ndim, stage_max);
-    if (ndim == 3)
-        hipLaunchKernelGGL(k_batch_rebuild<3>, dim3(g), dim3(kRebuildThreads), lds, s, A,
-                           (const double4 *)pos, (const double4 *)alt, count_rebuild, skin,
-                           type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx, cstart,
-                           (double4 *)xs, dsort, flags, which, npairs, gflag, work, stage_max);
-    else
-        hipLaunchKernelGGL(k_batch_rebuild<2>, dim3(g), dim3(kRebuildThreads), lds, s, A,
-                           (const double2 *)pos, (const double2 *)alt, count_rebuild, skin,
-                           type_bits, tpa, cap, npad, nbr, ncount, keys, cellidx, cstart,
-                           (double2 *)xs, dsort, flags, which, npairs, gflag, work, stage_max);
+                       force, A.geom, work, wslice);
+    const int gs = sort_grid < nsys ? sort_grid : nsys;
+    if (ndim == 3) {
+        hipLaunchKernelGGL(k_batch_sort<3>, dim3(gs), dim3(kSortThreads), 0, s, A,
+                           (const double4 *)pos, count_rebuild, keys, cellidx, cstart,
+                           (double4 *)xs, dsort, flags, which, nent, work);
+        hipLaunchKernelGGL(k_batch_build<3>, dim3(build_grid), dim3(kBuildSlice), 0, s, A,
+                           (const double4 *)alt, skin, type_bits, tpa, cap, npad, nbr, ncount,
+                           keys, cellidx, cstart, (const double4 *)xs, dsort, nent, gflag,
+                           work, wslice);
+    } else {
+        hipLaunchKernelGGL(k_batch_sort<2>, dim3(gs), dim3(kSortThreads), 0, s, A,
+                           (const double2 *)pos, count_rebuild, keys, cellidx, cstart,
+                           (double2 *)xs, dsort, flags, which, nent, work);
+        hipLaunchKernelGGL(k_batch_build<2>, dim3(build_grid), dim3(kBuildSlice), 0, s, A,
+                           (const double2 *)alt, skin, type_bits, tpa, cap, npad, nbr, ncount,
+                           keys, cellidx, cstart, (const double2 *)xs, dsort, nent, gflag,
+                           work, wslice);
+    }
     return hipGetLastError();
 }
 

@@ -86,9 +86,10 @@ struct parm_batch {
     double *dsort = nullptr;   // list diameters in the same order
     uint32_t *flags = nullptr, *which = nullptr, *gflag = nullptr, *hgflag = nullptr;
     int32_t *work = nullptr;   // worklist of systems to rebuild (count first)
-    int rgrid = 1;             // workgroups of the rebuild launch
-    int stage_max = 0;         // systems this small rebuild from LDS
-    unsigned long long *npairs = nullptr;
+    int32_t *wslice = nullptr; // first build slice of each listed system, then the total
+    int sort_grid = 1;         // workgroups of the rebuild's sort launch
+    int build_grid = 1;        // workgroups of the rebuild's list-build launch
+    unsigned long long *nent = nullptr;  // list entries of each system (twice its pairs)
 
     int pair_kind = kPairNone, ntypes = 0, single = 1, harmonic = 0;
     PairParam *tab = nullptr;
@@ -134,6 +135,7 @@ struct parm_batch {
         Lv.type_bits = type_bits;
         Lv.tpa = tpa;
         Lv.built = built ? 1 : 0;
+        Lv.wcap = tpa > 0 ? cap / tpa : 0;
         return Lv;
     }
     PairView pairview() const {
@@ -179,9 +181,10 @@ void bdestroy(parm_batch *b) {
     bfree(b->cstart);
     bfree(b->flags);
     bfree(b->work);
+    bfree(b->wslice);
     bfree(b->which);
     bfree(b->gflag);
-    bfree(b->npairs);
+    bfree(b->nent);
     bfree(b->tab);
     bfree(b->eatom);
     bfree(b->sysout);
@@ -283,8 +286,8 @@ int brebuild(parm_batch *b, bool force, bool count, const void *pos,
     PB_HIP(launch_batch_rebuild(b->ndim, b->nsys, b->arrays(), pos, alt, force ? 1 : 0,
                                 count ? 1 : 0, b->skin, b->type_bits, b->tpa, b->cap,
                                 b->npad, b->nbr, b->ncount, b->keys, b->cellidx, b->cstart,
-                                b->xs, b->dsort, b->flags, b->which, b->npairs, b->gflag, b->work, b->rgrid, b->stage_max,
-                                b->stream));
+                                b->xs, b->dsort, b->flags, b->which, b->nent, b->gflag,
+                                b->work, b->wslice, b->sort_grid, b->build_grid, b->stream));
     return PARM_OK;
 }
 
@@ -518,18 +521,10 @@ int parm_batch_create(int ndim, int precision, int64_t nsys, const int64_t *nato
         return bail(bfail(PARM_E_DEVICE, "this library is built for gfx950 only"));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(bfail(PARM_E_DEVICE, "stream creation failed"));
-    // rebuild launch: as many systems staged in LDS as a workgroup's LDS
-    // holds next to the cell counters (all of them, up to 4096 atoms, on a
-    // 160 KB part), one workgroup per CU at a time
-    {
-        const size_t per = (size_t)(ndim + 1) * sizeof(double);
-        const size_t fixed = batch_rebuild_static_lds() + 1024;
-        const size_t lim = (size_t)prop.sharedMemPerBlock;
-        b->stage_max = lim > fixed ? (int)std::min<size_t>(4096, (lim - fixed) / per) : 0;
-        if (const char *e = std::getenv("PARM_BATCH_STAGE"))
-            b->stage_max = std::min(b->stage_max, std::max(0, std::atoi(e)));
-        b->rgrid = std::max(1, prop.multiProcessorCount) * (b->stage_max > 1024 ? 1 : 2);
-    }
+    // rebuild launches: the sort one 1024-thread workgroup per CU, the list
+    // build eight 256-thread workgroups per CU (both loop over their work)
+    b->sort_grid = std::max(1, prop.multiProcessorCount);
+    b->build_grid = 8 * std::max(1, prop.multiProcessorCount);
     const size_t n = (size_t)tot;
     int rc;
     if ((rc = balloc((char **)&b->pos, n * b->pos_bytes)) ||
@@ -546,8 +541,8 @@ int parm_batch_create(int ndim, int precision, int64_t nsys, const int64_t *nato
         (rc = balloc(&b->keys, n)) || (rc = balloc(&b->cellidx, n)) ||
         (rc = balloc((char **)&b->xs, n * b->pos_bytes)) || (rc = balloc(&b->dsort, n)) ||
         (rc = balloc(&b->flags, (size_t)nsys)) || (rc = balloc(&b->which, (size_t)nsys)) ||
-        (rc = balloc(&b->work, (size_t)nsys + 1)) ||
-        (rc = balloc(&b->gflag, 1)) || (rc = balloc(&b->npairs, (size_t)nsys)) ||
+        (rc = balloc(&b->work, (size_t)nsys + 1)) || (rc = balloc(&b->wslice, (size_t)nsys + 1)) ||
+        (rc = balloc(&b->gflag, 1)) || (rc = balloc(&b->nent, (size_t)nsys)) ||
         (rc = balloc(&b->eatom, 4 * n)) || (rc = balloc(&b->sysout, 4 * (size_t)nsys)) ||
         (rc = balloc(&b->geom, (size_t)nsys)))
         return bail(rc);
@@ -566,7 +561,7 @@ int parm_batch_create(int ndim, int precision, int64_t nsys, const int64_t *nato
     (void)hipMemsetAsync(b->type, 0, n * sizeof(int32_t), st);
     (void)hipMemsetAsync(b->flags, 0, nsys * sizeof(uint32_t), st);
     (void)hipMemsetAsync(b->which, 0, nsys * sizeof(uint32_t), st);
-    (void)hipMemsetAsync(b->npairs, 0, nsys * sizeof(unsigned long long), st);
+    (void)hipMemsetAsync(b->nent, 0, nsys * sizeof(unsigned long long), st);
     (void)hipMemsetAsync(b->gflag, 0, sizeof(uint32_t), st);
     std::vector<double> ones(n, 1.0);
     if (hipMemcpyAsync(b->m, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, st) !=
@@ -858,11 +853,11 @@ int parm_batch_lists(parm_batch *b, uint32_t *which, int64_t *npairs) {
         PB_HIP(hipMemcpyAsync(which, b->which, sizeof(uint32_t) * b->nsys,
                               hipMemcpyDeviceToHost, b->stream));
     std::vector<unsigned long long> np(b->nsys);
-    PB_HIP(hipMemcpyAsync(np.data(), b->npairs, sizeof(unsigned long long) * b->nsys,
+    PB_HIP(hipMemcpyAsync(np.data(), b->nent, sizeof(unsigned long long) * b->nsys,
                           hipMemcpyDeviceToHost, b->stream));
     PB_HIP(hipStreamSynchronize(b->stream));
     if (npairs)
-        for (int s = 0; s < b->nsys; s++) npairs[s] = (int64_t)np[s];
+        for (int s = 0; s < b->nsys; s++) npairs[s] = (int64_t)(np[s] / 2);
     return PARM_OK;
 }
 

@@ -266,18 +266,19 @@ hipError_t launch_batch_drift(int ndim, bool damped, const BatchArrays &A,
 hipError_t launch_batch_disp(int ndim, const BatchArrays &A, hipStream_t s);
 hipError_t launch_batch_skin(int nsys, const BatchArrays &A, double skin, uint32_t *flags,
                              uint32_t *gflag, hipStream_t s);
+// Rebuilds the systems whose skin flag is set (all with force): worklist,
+// sort, then the list build spread over slices of the listed systems.
+// work: nsys + 1 ints (count, then system ids); wslice: nsys + 1 ints (each
+// listed system's first build slice, then the total).  nent: per system the
+// list entries of its last build (twice its pairs).
 hipError_t launch_batch_rebuild(int ndim, int nsys, const BatchArrays &A, const void *pos,
                                 const void *alt, int force, int count_rebuild, double skin,
                                 int type_bits, int tpa, int cap, int npad, uint32_t *nbr,
                                 int32_t *ncount, int32_t *keys, int32_t *cellidx,
                                 int32_t *cstart, void *xs, double *dsort, uint32_t *flags,
-                                uint32_t *which, unsigned long long *npairs, uint32_t *gflag,
-                                int32_t *work, int grid, int stage_max, hipStream_t s);
-// LDS of a rebuild workgroup: systems of up to stage_max atoms keep their
-// sorted positions and diameters in dynamic LDS; the static part holds the
-// cell counters.  work: nsys + 1 ints (the worklist).
-size_t batch_rebuild_lds(int ndim, int stage_max);
-size_t batch_rebuild_static_lds();
+                                uint32_t *which, unsigned long long *nent, uint32_t *gflag,
+                                int32_t *work, int32_t *wslice, int sort_grid, int build_grid,
+                                hipStream_t s);
 hipError_t launch_batch_sys_sums(int nsys, const BatchArrays &A, const double *eatom,
                                  double *out, hipStream_t s);
 constexpr int kBatchMaxCellsPerSystem = 4096;

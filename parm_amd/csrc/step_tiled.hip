@@ -72,15 +72,43 @@ __device__ __forceinline__ void lj64x(const PairParam &pp, const Vec<D> &r,
     }
 }
 
-// Minimum-image offset of x from the origin o, x - o - L rint((x - o) / L):
-// the staged offsets need not round like the reference's remainder (they are
-// differenced again and carry ~1e-16 relative error anyway).
+// x folded into the primary image, x - L rint(x / L): one rounding, of
+// magnitude ulp(L / 2), whatever image the unwrapped x is in.
 template <int D>
-__device__ __forceinline__ Vec<D> offset_from(const Vec<D> &x, const Vec<D> &o, const Box &b) {
+__device__ __forceinline__ Vec<D> fold_image(const Vec<D> &x, const Box &b) {
+    Vec<D> r;
+#pragma unroll
+    for (int k = 0; k < D; k++) r.c[k] = fma(-b.L[k], rint(x.c[k] * b.invL[k]), x.c[k]);
+    return r;
+}
+
+// Minimum-image offset of x from the origin o (o already folded):
+// d - L rint(d / L) with d = x - o.  `far` is set when x lies more than a box
+// length from o along some dimension: x - o then rounds at its own
+// magnitude, and offset_folded (x folded into the primary image first, every
+// rounding at ulp(L)) should be taken instead.  The staged offsets need not
+// round like the reference's remainder (they are differenced again).
+template <int D>
+__device__ __forceinline__ Vec<D> offset_from(const Vec<D> &x, const Vec<D> &o, const Box &b,
+                                              bool &far) {
     Vec<D> r;
 #pragma unroll
     for (int k = 0; k < D; k++) {
         const double d = x.c[k] - o.c[k];
+        far |= fabs(d) > b.L[k];
+        r.c[k] = fma(-b.L[k], rint(d * b.invL[k]), d);
+    }
+    return r;
+}
+
+template <int D>
+__device__ __forceinline__ Vec<D> offset_folded(const Vec<D> &x, const Vec<D> &o,
+                                                const Box &b) {
+    const Vec<D> xf = fold_image<D>(x, b);
+    Vec<D> r;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const double d = xf.c[k] - o.c[k];
         r.c[k] = fma(-b.L[k], rint(d * b.invL[k]), d);
     }
     return r;
@@ -158,7 +186,7 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     Vec<D> xo;
 #pragma unroll
     for (int k = 0; k < D; k++) xo.c[k] = 0.0;
-    if (!RAW && S > 0) xo = PosStore<D>::load(pos[a0]);
+    if (!RAW && S > 0) xo = fold_image<D>(PosStore<D>::load(pos[a0]), box);
     int wide = 0;  // some offset reaches L/4: pair differences need the fold
     int r = 0;     // run of slot s (slots rise along the loop)
     // kStageBatch slots per thread at a time: their atom indices first, then
@@ -193,7 +221,12 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
         const int tj = tb[q];
         Vec<D> v = xj;
         if (!RAW) {
-            v = offset_from<D>(xj, xo, box);
+            bool far = false;
+            v = offset_from<D>(xj, xo, box, far);
+            // atoms several box lengths out are rare: one wave-uniform test
+            if (__ballot(far) != 0ull) {
+                if (far) v = offset_folded<D>(xj, xo, box);
+            }
 #pragma unroll
             for (int k = 0; k < D; k++) wide |= fabs(v.c[k]) >= 0.5 * box.halfL[k];
             if (!COMPACT) stj[s] = j;
@@ -251,7 +284,7 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
 #pragma unroll
         for (int k = 0; k < D; k++) ri[k] = Lf[k] = hLf[k] = 0.0f;
         if (!RAW) {
-            ri64 = offset_from<D>(xi, xo, box);
+            ri64 = offset_folded<D>(xi, xo, box);
 #pragma unroll
             for (int k = 0; k < D; k++) {
                 ri[k] = (float)ri64.c[k];

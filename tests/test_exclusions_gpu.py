@@ -65,9 +65,9 @@ def test_ignored_pairs_leave_the_list(monkeypatch, ndim, n, tiled, tpa):
     # and nothing moved since: no further rebuild
     assert not s.update_list(False)
     s.set_forces(False)
-    ref = O.forces(x, L, O.PairTable(O.REPULSION, p["eps"], p["sigma"], p["third"],
-                                     p["types"]), f_ref, l_ref)
-    assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
+    table = O.PairTable(O.REPULSION, p["eps"], p["sigma"], p["third"], p["types"])
+    ref = O.forces(x, L, table, f_ref, l_ref)
+    S.assert_fp64_forces(s.forces(), ref, x, L, table, f_ref, l_ref)
     s.close()
 
 

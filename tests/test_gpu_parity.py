@@ -100,8 +100,7 @@ def test_fp64_forces_energy_virial(kind, ndim, n, rho, skin, bi):
     f_dev = s.forces()
     first, last = O.nlist(x, L, p["diam"], skin)
     ref = O.forces(x, L, _table(kind, p), first, last)
-    err = S.normalised_force_error(f_dev, ref["f"], ref["fabs"])
-    assert err <= 1e-12, err
+    S.assert_fp64_forces(f_dev, ref, x, L, _table(kind, p), first, last)
     sums = s.pair_sums()
     assert abs(sums["U"] - ref["U"]) <= 1e-12 * max(abs(ref["U"]), 1e-300) + 1e-13
     assert abs(sums["W"] - ref["W"]) <= 1e-12 * max(abs(ref["W"]), 1e-300) + 1e-13
@@ -121,7 +120,7 @@ def test_repulsion_general_exponent_and_multi_type_lj():
     s.set_forces(True)
     first, last = O.nlist(x, L, p["diam"], 0.3)
     ref = O.forces(x, L, _table(O.REPULSION, p), first, last)
-    assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
+    S.assert_fp64_forces(s.forces(), ref, x, L, _table(O.REPULSION, p), first, last)
     # two-type LJ with an eps <= 0 (purely repulsive) type pair
     types = (np.arange(n) % 2).astype(np.int32)
     q = dict(eps=[[1.0, 0.7], [0.7, -1.5]], sigma=[1.0, 1.2], third=[2.5, 2.2],
@@ -131,7 +130,7 @@ def test_repulsion_general_exponent_and_multi_type_lj():
     s2.set_forces(False)
     first, last = O.nlist(x, L, q["diam"], 0.3)
     ref = O.forces(x, L, _table(O.LJ_ATTRACT_REPULSE, q), first, last)
-    assert S.normalised_force_error(s2.forces(), ref["f"], ref["fabs"]) <= 1e-12
+    S.assert_fp64_forces(s2.forces(), ref, x, L, _table(O.LJ_ATTRACT_REPULSE, q), first, last)
     assert abs(s2.pair_sums()["U"] - ref["U"]) <= 1e-12 * abs(ref["U"])
 
 
@@ -385,7 +384,7 @@ def test_every_lanes_per_atom_layout(tpa, monkeypatch):
     s.set_forces(False)
     first, last = O.nlist(x, L, p["diam"], 0.3)
     ref = O.forces(x, L, _table(O.LJ_ATTRACT_REPULSE, p), first, last)
-    assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
+    S.assert_fp64_forces(s.forces(), ref, x, L, _table(O.LJ_ATTRACT_REPULSE, p), first, last)
     f_dev, l_dev = s.pairs()
     assert np.array_equal(f_dev, first) and np.array_equal(l_dev, last)
     assert abs(s.pair_sums()["U"] - ref["U"]) <= 1e-12 * abs(ref["U"])
@@ -415,7 +414,7 @@ def test_rebuild_after_denser_packing_replans():
         assert np.array_equal(f_dev, first) and np.array_equal(l_dev, last)
         s.set_forces(False)
         ref = O.forces(xs, L, t, first, last)
-        assert S.normalised_force_error(s.forces(), ref["f"], ref["fabs"]) <= 1e-12
+        S.assert_fp64_forces(s.forces(), ref, xs, L, t, first, last)
 
 
 @pytest.mark.parametrize("case", ["empty", "no_members", "single"])

@@ -254,3 +254,54 @@ def test_nlcg_reset_and_descend():
     Lfac = math.exp(1e-3 * fl / (10.0 * 3))
     assert np.allclose(s.box(), L0 * Lfac, rtol=1e-15, atol=0)
     assert np.allclose(x1, x0 * Lfac + 1e-3 * f1, rtol=1e-14, atol=1e-14)
+
+
+def _anchored_chain(seed, kmax):
+    """Four atoms on a line between two fixed anchors, every pair inside its
+    harmonic-repulsion contact: while the order along the line holds, the
+    energy is an exact quadratic in the four mobile x coordinates (positive
+    definite through the strong anchors).  The box is frozen: kappa = 1e30
+    makes Lfac = exp(dx vl / (kappa NDIM)) exactly 1."""
+    n = 6
+    rng = np.random.default_rng(seed)
+    x = np.zeros((n, 3))
+    x[:, 0] = [0.0, 1.2, 2.4, 3.6, 4.8, 6.0]
+    x[1:5, 0] += rng.uniform(-0.1, 0.1, 4)
+    sig = np.array([7.0, 7.5, 8.0, 8.5, 9.0, 7.2])
+    eps = np.array([50.0, 2.0, 0.5, 1.5, 3.0, 50.0])
+    m = np.array([np.inf, 1.0, 1.0, 1.0, 1.0, np.inf])
+    table = O.PairTable(O.REPULSION, eps, sig, np.full(n, 2.0), np.arange(n, dtype=np.int32))
+    s = O.System(m, np.full(3, 100.0), table, sig, 0.5, ndim=3)
+    s.set_state(x=x, v=np.zeros_like(x), a=np.zeros_like(x))
+    s.initialize()
+    c = O.NLCG(s, 1e-3, 0.0, kappa=1e30, kmax=kmax)
+    c.set_limits(alphamax=0.0, afrac=0.0, dxmax=0.0, stepmax=0.0, maxdV=0.0)
+    c.reset()
+    return s, c
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_nlcg_conjugate_gradient_terminates_on_a_quadratic(seed):
+    """On an exact quadratic in d unknowns, conjugate gradients with exact
+    line searches reach the minimum in at most d steps; the secant search
+    is exact on a quadratic (its derivative along the line is linear).  So
+    CollectionNLCG::timestep (src/collection.cpp:685-835) must bring the
+    forces to rounding level in 4 steps here, while the same steps with the
+    direction reset every step (kmax = 1: steepest descent) must not: this
+    pins the secant step, the Polak-Ribiere beta's denominator and its
+    f_new.f_new term, and v = a + beta v.  (On a quadratic with exact line
+    searches f_new.f_old = 0, so the beta numerator's -f_new.f_old term is
+    not pinned by this test.)"""
+    def fmax(sys):  # the mobile atoms' forces (the anchors' are not minimised)
+        return np.abs(sys.state()[3][1:5]).max()
+
+    s, c = _anchored_chain(seed, kmax=1000)
+    f0 = fmax(s)
+    c.timestep(3)
+    assert fmax(s) > 1e-6 * f0
+    c.timestep(1)
+    assert fmax(s) < 1e-13 * f0
+    assert np.all(np.diff(s.state()[0][:, 0]) > 0)  # the order held: the quadratic was exact
+    s2, c2 = _anchored_chain(seed, kmax=1)
+    c2.timestep(4)
+    assert fmax(s2) > 1e-6 * f0

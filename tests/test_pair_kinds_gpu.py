@@ -90,11 +90,7 @@ def test_pair_kind_forces_energy_and_pairs(kind, ndim, n, rho, skin):
     assert np.array_equal(f_dev, first) and np.array_equal(l_dev, last)
     s.set_forces(False)
     ref = O.forces(x, L, table, first, last)
-    # FP64 rounds each term of a pair force, not their (cancelling) sum: the
-    # bar adds 1e-13 of the terms' scale to 1e-12 of sum_j |f_ij|.
-    d = np.sqrt(((s.forces() - ref["f"]) ** 2).sum(1))
-    bar = 1e-12 * ref["fabs"] + 1e-13 * _term_scale(kind, x, L, table, first, last)
-    assert np.all(d <= bar), float(np.max(d / np.maximum(bar, 1e-300)))
+    S.assert_fp64_forces(s.forces(), ref, x, L, table, first, last)
     sums = s.pair_sums()
     assert abs(sums["U"] - ref["U"]) <= 1e-12 * abs(ref["U"]) + 1e-13
     assert abs(sums["W"] - ref["W"]) <= 1e-12 * abs(ref["W"]) + 1e-13
@@ -125,33 +121,6 @@ def test_pair_kind_verlet_trajectory(kind, ndim, n, rho, skin):
     s.close()
 
 
-def _term_scale(kind, x, L, table, first, last):
-    """Per atom, the sum over its pairs of the scale of the terms FP32
-    rounds: 12 eps s (s + 1) / r for the LJ forms (s = (sigma / r)^6, the two
-    terms of the force, which cancel near the minimum), eps / sigma for the
-    repulsion (whose 1 - R/sigma cancels near contact)."""
-    t1, t2 = table.types[first], table.types[last]
-    if table.eps.ndim == 2:
-        eps = table.eps[t1, t2]
-    else:
-        eps = np.sqrt(table.eps[t1] * table.eps[t2])
-    sig = table.sigma[t1, t2] if table.sigma.ndim == 2 else \
-        (table.sigma[t1] + table.sigma[t2]) / 2
-    d = x[first] - x[last]
-    d -= L * np.round(d / L)
-    r = np.sqrt((d * d).sum(1))
-    if kind == O.REPULSION_II:
-        t = np.where(r < sig, eps / sig, 0.0)
-    else:
-        cut = 1.0 if kind == O.LJ_REPULSE else np.maximum(table.third[t1], table.third[t2])
-        s6 = (sig / r) ** 6
-        t = np.where(r < cut * sig, 12 * eps * s6 * (s6 + 1) / r, 0.0)
-    out = np.zeros(len(x))
-    np.add.at(out, first, t)
-    np.add.at(out, last, t)
-    return out
-
-
 @pytest.mark.parametrize("kind", [O.LJ_CUT, O.LJ_REPULSE, O.REPULSION_II])
 def test_pair_kind_mixed_precision_forces(kind):
     """Mixed precision: FP32 pair arithmetic, so the per-atom error is bounded
@@ -164,7 +133,7 @@ def test_pair_kind_mixed_precision_forces(kind):
     first, last = O.nlist(x, L, diam, 0.3)
     ref = O.forces(x, L, table, first, last)
     d = np.sqrt(((s.forces() - ref["f"]) ** 2).sum(1))
-    bar = 2e-5 * ref["fabs"] + 1e-6 * _term_scale(kind, x, L, table, first, last)
+    bar = 2e-5 * ref["fabs"] + 1e-6 * S.pair_term_scale(x, L, table, first, last)
     assert np.all(d <= bar), float(np.max(d / np.maximum(bar, 1e-300)))
     s.close()
 
