@@ -302,6 +302,17 @@ void make_grid(parm_system *s) {
     s->grid = g;
 }
 
+// Per-lane entry counts are 16-bit (lcount): a list whose capacity could
+// give one lane more than 65535 entries is refused, not wrapped.
+int check_lane_counts(const parm_system *s) {
+    if ((int64_t)s->cap > (int64_t)65535 * s->tpa)
+        return fail(PARM_E_UNSUPPORTED,
+                    "more than 65535 list entries per lane (capacity " + std::to_string(s->cap) +
+                        " at " + std::to_string(s->tpa) +
+                        " lanes per atom): use more lanes per atom or a smaller skin");
+    return PARM_OK;
+}
+
 int ensure_list_capacity(parm_system *s, int cap) {
     if (cap <= s->cap && s->nbr) return PARM_OK;
     free_list(s);
@@ -425,6 +436,7 @@ int plan_tiles(parm_system *s) {
 // Builds the list at positions `at` (the current ones, or xref for a
 // relayout) in the layout chosen by accept_tiles.
 int build_list(parm_system *s, const void *at) {
+    PARM_CHECK(check_lane_counts(s));
     hipStream_t st = s->stream;
     if (s->tile_ok) {
         // the tiled build writes either list layout
@@ -487,6 +499,7 @@ int build_checked(parm_system *s, const void *at) {
 // plan fits the bounds (and the list its capacity) the build stands;
 // otherwise the plan is accepted as read and the list built again.
 int build_speculative(parm_system *s) {
+    PARM_CHECK(check_lane_counts(s));
     const int pstride = force_blocks(std::max(s->n, 1), 8);
     const int grid = std::min(s->tiles_cap, pstride);
     // the largest stencil whose staging fits the LDS a workgroup may take

@@ -850,18 +850,22 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                                 }
                             }
                         }
-                        int np = 0;
+                        uint32_t bits = 0;
 #pragma unroll
-                        for (int u = 0; u < U; u++) np += pass[u] ? 1 : 0;
+                        for (int u = 0; u < U; u++) bits |= pass[u] ? (1u << u) : 0u;
+                        const int np = __builtin_popcount(bits);
                         // exclusive prefix of the round's passes over the group
                         // (the G lanes of one atom), and the group total
                         int incl, tot;
                         group_scan<G>(np, sub, glast, incl, tot);
                         int q = cnt + incl - np;
                         cnt += tot;
-#pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            if (!pass[u]) continue;
+                        // the lane's passes in slot order, one store each: the
+                        // wavefront runs as many store rounds as its busiest
+                        // lane has passes (a few), not one per candidate
+                        while (bits) {
+                            const int u = __builtin_ctz(bits);
+                            bits &= bits - 1u;
                             if (q < cap) {
                                 const int s = base + u * G + sub;
                                 if (OUT32)
