@@ -60,8 +60,8 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE by 1.0, the factors that make the same counters read 1 GiB for a
 # 1 GiB device copy (profiles/k_force_lj_1M_pmc.txt).
 TRAFFIC_BYTES_PER_ATOM = {  # (tiled list, lanes per atom) -> bytes per atom
-    (True, 1): 401.5, (True, 2): 401.6, (True, 4): 405.3, (True, 8): 416.5,
-    (False, 1): 559.0, (False, 2): 546.7, (False, 4): 563.3, (False, 8): 571.9,
+    (True, 1): 401.6, (True, 2): 400.9, (True, 4): 404.3, (True, 8): 416.0,
+    (False, 1): 558.9, (False, 2): 546.4, (False, 4): 563.3, (False, 8): 571.9,
 }
 
 
