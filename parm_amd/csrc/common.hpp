@@ -161,6 +161,37 @@ __device__ __forceinline__ double rcp64(double x) {
 }
 
 // ------------------------------------------------------------ reductions
+// DPP lane moves (VALU operand modifiers: no trip through the LDS crossbar).
+// A 64-bit value moves as its two 32-bit halves.  The patterns used here
+// read a valid source lane for every lane, so every lane is written.
+constexpr int kDppQuadXor1 = 0xB1;       // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int kDppQuadXor2 = 0x4E;       // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int kDppRowHalfMirror = 0x141; // lane 7 - l within each 8 lanes
+constexpr int kDppRowMirror = 0x140;     // lane 15 - l within each 16 lanes
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(
+        (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 4 lanes of each quad (every lane of the quad gets it), in the
+// order of the xor-1 then xor-2 butterfly: (v0 + v1) + (v2 + v3).
+__device__ __forceinline__ double quad_sum(double v) {
+    v += dpp_f64<kDppQuadXor1>(v);
+    return v + dpp_f64<kDppQuadXor2>(v);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -216,15 +247,48 @@ struct Top2 {
     }
 };
 
+// (a, b) := the top 2 of {a, b, a2, b2} (a >= b, a2 >= b2; no NaN: Top2
+// never holds one).
+__device__ __forceinline__ void top2_merge(double &a, double &b, double a2, double b2) {
+    const double lo = fmin(a, a2);
+    a = fmax(a, a2);
+    b = fmax(lo, fmax(b, b2));
+}
+
+template <int CTRL>
+__device__ __forceinline__ void top2_dpp_step(Top2 &t) {
+    const double a2 = dpp_f64<CTRL>(t.a), b2 = dpp_f64<CTRL>(t.b);
+    top2_merge(t.a, t.b, a2, b2);
+}
+
+// Top 2 over the wavefront (all lanes active), returned to every lane: DPP
+// steps leave each row of 16 lanes with its row's top 2, then the four rows
+// are merged from lanes 0, 16, 32, 48.  The top 2 of a multiset does not
+// depend on the merge order.
 __device__ __forceinline__ Top2 wave_top2(Top2 t) {
+    top2_dpp_step<kDppQuadXor1>(t);
+    top2_dpp_step<kDppQuadXor2>(t);
+    top2_dpp_step<kDppRowHalfMirror>(t);
+    top2_dpp_step<kDppRowMirror>(t);
+    Top2 r;
+    r.a = readlane_f64(t.a, 0);
+    r.b = readlane_f64(t.b, 0);
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        Top2 o;
-        o.a = __shfl_xor(t.a, off);
-        o.b = __shfl_xor(t.b, off);
-        t.merge(o);
-    }
-    return t;
+    for (int row = 1; row < kWave / 16; row++)
+        top2_merge(r.a, r.b, readlane_f64(t.a, 16 * row), readlane_f64(t.b, 16 * row));
+    return r;
+}
+
+// Sum over the wavefront of small non-negative integer counts held as
+// doubles (exact in any order), returned to every lane.
+__device__ __forceinline__ double wave_count(double v) {
+    v = quad_sum(v);
+    v += dpp_f64<kDppRowHalfMirror>(v);
+    v += dpp_f64<kDppRowMirror>(v);
+    double r = readlane_f64(v, 0);
+#pragma unroll
+    for (int row = 1; row < kWave / 16; row++) r += readlane_f64(v, 16 * row);
+    return r;
 }
 
 // Flag word stored by the reduction kernels in device memory and in a pinned

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kFinal) void k_finalize(int nparts, ReduceBuf R) {
             nf += R.part[kNonFinite * ps + b];
         }
         bt = wave_top2(bt);
-        nf = wave_sum(nf);
+        nf = wave_count(nf);
         __shared__ double fin2[NW][3];
         if (lane == 0) {
             fin2[w][0] = bt.a;

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
     constexpr bool COMPACT = SINGLE && !RAW;
     constexpr int NT = kWave * TPA;
     extern __shared__ double4 st64[];
-    __shared__ int2 runs[kTileRuns];
+    __shared__ int2 runs[kRunSlots];  // run_entry form (tiles.hpp)
     __shared__ int4 hdr;
     float4 *st32 = reinterpret_cast<float4 *>(st64);
     double *stc64 = reinterpret_cast<double *>(st64);
@@ -144,8 +144,7 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                          : reinterpret_cast<int32_t *>(st64 + Lv.smax);
 
     const int t = xcd_block(blockIdx.x, gridDim.x);
-    if (threadIdx.x < kTileRuns)
-        runs[threadIdx.x] = Lv.truns[(size_t)t * kTileRuns + threadIdx.x];
+    if (threadIdx.x < kRunSlots) runs[threadIdx.x] = run_entry(Lv.truns, t, threadIdx.x);
     if (threadIdx.x == 0) hdr = Lv.tiles[t];
     __syncthreads();
     const int a0 = hdr.x, tcnt = hdr.y;
@@ -183,27 +182,27 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                 if (q < Lv.wcap) cur[q] = wp[(size_t)q * wstride];
         }
     }
+    // the tile origin: loaded now, folded once the stencil's loads are issued
+    PT xo_raw{};
+    if (!RAW && S > 0) xo_raw = pos[a0];
     Vec<D> xo;
 #pragma unroll
     for (int k = 0; k < D; k++) xo.c[k] = 0.0;
-    if (!RAW && S > 0) xo = fold_image<D>(PosStore<D>::load(pos[a0]), box);
+    bool xo_ready = false;
     int wide = 0;  // some offset reaches L/4: pair differences need the fold
-    int r = 0;     // run of slot s (slots rise along the loop)
-    // kStageBatch slots per thread at a time: their atom indices first, then
-    // all their position (and type) loads in flight together, then the stores
-    constexpr int kStageBatch = 4;
+    // kStageBatch slots per thread at a time: their atom indices first (each
+    // an independent binary lifting over the runs), then all their position
+    // (and type) loads in flight together, then the stores
+    constexpr int kStageBatch = 5;
     for (int s0 = threadIdx.x; s0 < S; s0 += kStageBatch * NT) {
         int jb[kStageBatch];
         PT pb[kStageBatch];
         int tb[kStageBatch];
+        {
+            int sb[kStageBatch];
 #pragma unroll
-        for (int q = 0; q < kStageBatch; q++) {
-            const int s = s0 + q * NT;
-            jb[q] = 0;
-            if (s < S) {
-                while (r + 1 < kTileRuns && runs[r + 1].y <= s) r++;
-                jb[q] = runs[r].x + (s - runs[r].y);
-            }
+            for (int q = 0; q < kStageBatch; q++) sb[q] = s0 + q * NT;
+            slot_atoms<kStageBatch>(runs, sb, jb);
         }
 #pragma unroll
         for (int q = 0; q < kStageBatch; q++) {
@@ -211,6 +210,10 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
                 pb[q] = pos[jb[q]];
                 tb[q] = SINGLE ? 0 : A.type[jb[q]];
             }
+        }
+        if (!RAW && !xo_ready) {
+            xo = fold_image<D>(PosStore<D>::load(xo_raw), box);
+            xo_ready = true;
         }
 #pragma unroll
         for (int q = 0; q < kStageBatch; q++) {
@@ -261,6 +264,8 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
         }
         }
     }
+    // (a thread with no slot to stage still needs the origin)
+    if (!RAW && !xo_ready) xo = fold_image<D>(PosStore<D>::load(xo_raw), box);
     const bool fold = __syncthreads_or(wide) != 0;
 
     PairAcc acc;
@@ -296,8 +301,7 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
         auto exact_d2 = [&](int s) {
             int j;
             if constexpr (COMPACT) {
-                const int r = slot_run(runs, s);
-                j = runs[r].x + (s - runs[r].y);
+                j = slot_atom(runs, s);
             } else {
                 j = stj[s];
             }
@@ -504,10 +508,18 @@ __global__ __launch_bounds__(kWave *TPA) void k_force_tiled(AtomArrays A, ListVi
 #pragma unroll
         for (int k = 0; k < D; k++) fi[k] += (double)f32[k];
     }
+    // the atom's TPA lanes summed (xor butterfly; within a quad by DPP)
+    if constexpr (TPA == 2) {
 #pragma unroll
-    for (int o = 1; o < TPA; o <<= 1) {
+        for (int k = 0; k < D; k++) fi[k] += dpp_f64<kDppQuadXor1>(fi[k]);
+    } else if constexpr (TPA >= 4) {
 #pragma unroll
-        for (int k = 0; k < D; k++) fi[k] += __shfl_xor(fi[k], o);
+        for (int k = 0; k < D; k++) fi[k] = quad_sum(fi[k]);
+#pragma unroll
+        for (int o = 4; o < TPA; o <<= 1) {
+#pragma unroll
+            for (int k = 0; k < D; k++) fi[k] += __shfl_xor(fi[k], o);
+        }
     }
     // The integrator's per-atom memory traffic is done by the first
     // wavefront, one lane per atom (64 consecutive atoms: coalesced), with

@@ -51,7 +51,7 @@ __device__ __forceinline__ void block_partial(double (&v)[kRedWidth], Top2 t,
     }
     if (SKIN && (!SKIN_W0 || w == 0)) {
         t = wave_top2(t);
-        v[kNonFinite] = wave_sum(v[kNonFinite]);
+        v[kNonFinite] = wave_count(v[kNonFinite]);
     }
     if (lane == 0) {
         if (SUMS) {

@@ -250,17 +250,54 @@ __device__ __forceinline__ int slot_cell(const TileLayout &L, int s) {
     return lo;
 }
 
-// Run holding staged slot s: the last r with off(r) <= s, off(r) = runs[r].y.
-__device__ __forceinline__ int slot_run(const int2 *runs, int s) {
-    int lo = 0, hi = kTileRuns;  // runs[lo].y <= s, s < off(hi) (= S)
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (runs[mid].y <= s)
-            lo = mid;
-        else
-            hi = mid;
+// A tile's runs as the force kernels hold them in LDS: run r as (first atom -
+// first slot, first slot), padded to kRunSlots entries with (0, INT_MAX), so
+// that the atom of a staged slot is found by binary lifting with no bounds
+// checks: five independent-of-the-data steps, each one LDS read.
+constexpr int kRunSlots = 32;
+static_assert(kRunSlots >= kTileRuns && (kRunSlots & (kRunSlots - 1)) == 0,
+              "runs padded to a power of two");
+
+__device__ __forceinline__ int2 run_entry(const int2 *truns, int t, int r) {
+    if (r >= kTileRuns) return int2{0, 0x7fffffff};
+    const int2 u = truns[(size_t)t * kTileRuns + r];
+    return int2{u.x - u.y, u.y};
+}
+
+// Atoms of K staged slots s[q] (0 <= s[q] < S): s plus the offset of the last
+// run whose first slot is <= s (empty runs share their first slot with the
+// next run, which the lifting then takes).  The K liftings advance together,
+// one LDS read each per step, so their reads are in flight at once.
+template <int K>
+__device__ __forceinline__ void slot_atoms(const int2 *rs, const int (&s)[K], int (&j)[K]) {
+    int lo[K], d[K];
+    const int d0 = rs[0].x;
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        lo[q] = 0;
+        d[q] = d0;
     }
-    return lo;
+#pragma unroll
+    for (int step = kRunSlots / 2; step > 0; step >>= 1) {
+        int2 e[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) e[q] = rs[lo[q] + step];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const bool take = e[q].y <= s[q];
+            lo[q] = take ? lo[q] + step : lo[q];
+            d[q] = take ? e[q].x : d[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < K; q++) j[q] = s[q] + d[q];
+}
+
+__device__ __forceinline__ int slot_atom(const int2 *rs, int s) {
+    const int sv[1] = {s};
+    int j[1];
+    slot_atoms<1>(rs, sv, j);
+    return j[0];
 }
 
 // Position of list entry q (lane q % tpa, lane-local index m = q / tpa) in the
