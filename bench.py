@@ -18,7 +18,9 @@ steps.  Untimed beside them: 10^4-step NVE runs in both precisions (after a
 the steady-state throughput, and the per-atom force error of the mixed form.
 
 Roofline: the fused step kernel (forces + kick + next drift + skin test) is
-timed with HIP events on its own stream; its algorithmic bytes per launch are
+timed with HIP events on its own stream (the end event is recorded right
+after the kernel, before its k_finalize reduction launches); its algorithmic
+bytes per launch are
 N * (192 + 4 * nbar) with nbar the mean full-list length (FP64 state read and
 written once, one int32 list index per entry, neighbour gathers counted as
 cache hits).  Peak: 8 TB/s HBM3E.
@@ -60,8 +62,8 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE by 1.0, the factors that make the same counters read 1 GiB for a
 # 1 GiB device copy (profiles/k_force_lj_1M_pmc.txt).
 TRAFFIC_BYTES_PER_ATOM = {  # (tiled list, lanes per atom) -> bytes per atom
-    (True, 1): 401.6, (True, 2): 400.9, (True, 4): 404.3, (True, 8): 416.0,
-    (False, 1): 558.9, (False, 2): 546.4, (False, 4): 563.3, (False, 8): 571.9,
+    (True, 1): 401.6, (True, 2): 401.0, (True, 4): 404.5, (True, 8): 416.0,
+    (False, 1): 559.3, (False, 2): 546.2, (False, 4): 563.3, (False, 8): 571.9,
 }
 
 

@@ -660,11 +660,14 @@ hipEvent_t pool_event(parm_system *s) {
     return s->ev_pool[s->ev_used++];
 }
 
-// Brackets a launch with timing events when instrumentation is on.
+// Brackets a launch with timing events when instrumentation is on.  A force
+// launch given end() as its ReduceBuf::done records the end itself, right
+// after the force kernel (its k_finalize then falls outside the bracket).
 struct ProfScope {
     parm_system *s;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> *dst;
     hipEvent_t a = nullptr, b = nullptr;
+    bool callee_end = false;
     ProfScope(parm_system *s_, std::vector<std::pair<hipEvent_t, hipEvent_t>> *d)
         : s(s_), dst(d) {
         if (!s->prof) return;
@@ -672,9 +675,14 @@ struct ProfScope {
         b = pool_event(s);
         if (a && b) (void)hipEventRecord(a, s->stream);
     }
+    hipEvent_t end() {
+        if (!s->prof || !a || !b) return nullptr;
+        callee_end = true;
+        return b;
+    }
     ~ProfScope() {
         if (!s->prof || !a || !b) return;
-        (void)hipEventRecord(b, s->stream);
+        if (!callee_end) (void)hipEventRecord(b, s->stream);
         dst->emplace_back(a, b);
     }
 };
@@ -814,9 +822,11 @@ int run_steps(parm_system *s, bool damped, const StepCoef &c, int64_t nsteps) {
         if (!last) {
             {
                 ProfScope ps(s, &s->prof_step);
+                ReduceBuf R = s->redbuf(next);
+                R.done = ps.end();
                 PARM_HIP(launch_force(s->ndim, s->prec, kick_drift, false,
                                       s->arrays(true), s->listview(), s->pairview(),
-                                      s->box, c, s->redbuf(next), st));
+                                      s->box, c, R, st));
             }
             PARM_CHECK(post_flag(s, next));
         } else {

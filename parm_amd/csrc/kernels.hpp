@@ -195,6 +195,8 @@ struct ReduceBuf {
     uint32_t *flag;               // flag word
     double skin;                  // skin test threshold
     uint32_t *hflag;              // device-visible pinned host copy of the flag word, or null
+    hipEvent_t done;              // host side, or null: recorded on the launch stream right
+                                  // after the force kernel, before its k_finalize (timing)
 };
 constexpr int kRedWidth = 16;     // U, W, S[9], contacts, overlaps, top1, top2, nonfinite
 

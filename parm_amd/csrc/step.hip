@@ -353,6 +353,7 @@ static void launch_force_tt(bool energy, const AtomArrays &A, const ListView &Lv
     dim3 g(nb), b(kBlock);
     hipLaunchKernelGGL((k_force<D, KIND, PREC, SINGLE, HARM, STAGE, ENERGY, TPA>), g, b, 0,
                        s, A, Lv, P, box, c, R);
+    if (R.done) (void)hipEventRecord(R.done, s);
     if (ENERGY || DRIFT) launch_finalize<ENERGY, DRIFT>(nb, R, s);
 }
 

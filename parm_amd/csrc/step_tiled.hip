@@ -565,6 +565,7 @@ void tiled_tt(const AtomArrays &A, const ListView &Lv, const PairView &P, const 
     const size_t lds = tiled_force_lds(D, PREC, KIND, SINGLE, Lv.smax);
     hipLaunchKernelGGL((k_force_tiled<D, KIND, PREC, SINGLE, HARM, STAGE, TPA>),
                        dim3(Lv.ntiles), dim3(kWave * TPA), lds, s, A, Lv, P, box, c, R);
+    if (R.done) (void)hipEventRecord(R.done, s);
     if (ENERGY || DRIFT) (void)launch_finalize_rt(ENERGY, DRIFT, Lv.ntiles, R, s);
 }
 
