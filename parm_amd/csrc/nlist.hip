@@ -802,11 +802,24 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                             const int s = base + u * G + sub;
                             pw[u] = st_f[s];
                         }
-                        bool pass[U], unsure[U];
-                        bool anyu = false;
+                        // candidate u of this lane is slot base + u G + sub;
+                        // `valid`: the ones below hi that are not the atom
+                        // itself (one mask per round, not a test per slot)
+                        const int first = base + sub;
+                        const int nv = min(U, max(0, (hi - first + G - 1) / G));
+                        uint32_t valid = nv >= 32 ? ~0u : ((1u << nv) - 1u);
+                        {
+                            const int ds = self - first;
+                            if (ds >= 0 && (ds % G) == 0 && ds / G < U)
+                                valid &= ~(1u << (ds / G));
+                        }
+                        // bit u: d2 below the cutoff by more than the
+                        // tolerance (pbits), within it of the cutoff or below
+                        // (ubits); filled from the top so that bit u is
+                        // candidate u
+                        uint32_t pbits = 0, ubits = 0;
 #pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            const int s = base + u * G + sub;
+                        for (int u = U - 1; u >= 0; u--) {
                             float d2 = 0.0f;
 #pragma unroll
                             for (int k2 = 0; k2 < D; k2++) {
@@ -819,40 +832,38 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                                 const float cc = fmaf(0.5f, dif + pw[u].v[UNI ? 0 : D], skf);
                                 c2 = cc * cc;
                             }
-                            const bool ok = s < hi && s != self;
-                            pass[u] = ok && d2 < c2 - tol;
-                            unsure[u] = ok && !(d2 < c2 - tol) && d2 <= c2 + tol;
-                            anyu |= unsure[u];
+                            pbits = 2u * pbits + (d2 < c2 - tol ? 1u : 0u);
+                            ubits = 2u * ubits + (d2 <= c2 + tol ? 1u : 0u);
                         }
+                        pbits &= valid;
+                        ubits &= valid & ~pbits;
                         // candidates within the tolerance of the cutoff: the
                         // reference's exact predicate (one wave-uniform branch)
-                        if (__ballot(anyu)) {
-#pragma unroll
-                            for (int u = 0; u < U; u++) {
-                                if (unsure[u]) {
-                                    const int s = base + u * G + sub;
-                                    const int cj = slot_cell(L, s);
-                                    const int j = L.g0[cj] + (s - L.off[cj]);
-                                    pass[u] = in_list<D>(xi, PosStore<D>::load(pos[j]), di,
-                                                         diam[j], box, skin);
-                                }
+                        if (__ballot(ubits != 0u)) {
+                            while (ubits) {
+                                const int u = __builtin_ctz(ubits);
+                                ubits &= ubits - 1u;
+                                const int s = first + u * G;
+                                const int cj = slot_cell(L, s);
+                                const int j = L.g0[cj] + (s - L.off[cj]);
+                                if (in_list<D>(xi, PosStore<D>::load(pos[j]), di, diam[j], box,
+                                               skin))
+                                    pbits |= 1u << u;
                             }
                         }
                         // ignored pairs never enter the list (src/trackers.cpp:64)
                         if (ex.any) {
-#pragma unroll
-                            for (int u = 0; u < U; u++) {
-                                if (pass[u]) {
-                                    const int s = base + u * G + sub;
-                                    const int cj = slot_cell(L, s);
-                                    const int j = L.g0[cj] + (s - L.off[cj]);
-                                    pass[u] = !excluded(ex, oi, ex.orig[j]);
-                                }
+                            uint32_t b = pbits;
+                            while (b) {
+                                const int u = __builtin_ctz(b);
+                                b &= b - 1u;
+                                const int s = first + u * G;
+                                const int cj = slot_cell(L, s);
+                                const int j = L.g0[cj] + (s - L.off[cj]);
+                                if (excluded(ex, oi, ex.orig[j])) pbits &= ~(1u << u);
                             }
                         }
-                        uint32_t bits = 0;
-#pragma unroll
-                        for (int u = 0; u < U; u++) bits |= pass[u] ? (1u << u) : 0u;
+                        uint32_t bits = pbits;
                         const int np = __builtin_popcount(bits);
                         // exclusive prefix of the round's passes over the group
                         // (the G lanes of one atom), and the group total
