@@ -20,6 +20,7 @@
 //   k_tile_sizes<D>  staged slots of each tile, their maximum
 //   k_build_tiled<D> the tile's stencil staged in LDS, the same predicate
 //                    evaluated from LDS, 16-bit staged-slot entries
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     const typename PosStore<D>::T *__restrict__ xw, const int32_t *__restrict__ rank,
     const double *__restrict__ diam, const int32_t *__restrict__ type, int type_bits,
     const uint32_t *__restrict__ keys, const int32_t *__restrict__ cstart, CellGrid g, Box box,
-    double skin, double dmax, int tpa, int cap, int npad, void *__restrict__ nbr,
+    double skin, double dmax, int tpa, int cap, int npad, int off24, void *__restrict__ nbr,
     int32_t *__restrict__ ncount, uint16_t *__restrict__ lcount,
     typename PosStore<D>::T *__restrict__ xref, unsigned long long *__restrict__ part,
     uint32_t *__restrict__ flag, const int *__restrict__ spec, ExclView ex) {
@@ -668,7 +669,7 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     uint32_t *st_e = reinterpret_cast<uint32_t *>(st_f + smax + kBuildPad);  // OUT32 entries
     static_assert(G * U <= kBuildPad, "a round stays inside the padded stage");
     __shared__ TileLayout L;
-    __shared__ int2 runs[kTileRuns];
+    __shared__ int2 runs[kRunSlots];  // run_entry form (tiles.hpp)
     __shared__ double rimg[kTileRuns][3];
     __shared__ unsigned long long red[G][2];
     __shared__ int mbits;
@@ -699,9 +700,9 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     tile_layout<D>(tiles, t, keys, cstart, g, L);
     const int S = L.off[L.NR * L.W];
     // the stencil's runs (contiguous atoms, one periodic image each)
+    if (threadIdx.x < kRunSlots) runs[threadIdx.x] = run_entry(truns, t, threadIdx.x);
     if (threadIdx.x < kTileRuns) {
         const int r = threadIdx.x;
-        runs[r] = truns[(size_t)t * kTileRuns + r];
         int img[3];
         run_image<D>(L, g, r, img);
 #pragma unroll
@@ -713,20 +714,41 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     if (S > 0) wo = PosStore<D>::load(xw[L.a0]);
     __syncthreads();
     float mloc = 0.0f;
-    int r = 0;  // run of slot s (s grows along the loop: a monotone walk)
-    for (int s = threadIdx.x; s < S; s += NT) {
-        while (r + 1 < kTileRuns && runs[r + 1].y <= s) r++;
-        const int j = runs[r].x + (s - runs[r].y);
-        const Vec<D> w = PosStore<D>::load(xw[j]);
-        Slot sl;
+    // kStage slots per thread at a time: their atoms and runs found by binary
+    // lifting (the kStage liftings advancing together), then all their loads
+    // in flight together, then the stores
+    constexpr int kStage = 4;
+    for (int s0 = threadIdx.x; s0 < S; s0 += kStage * NT) {
+        int sb[kStage], rb[kStage], jb[kStage];
 #pragma unroll
-        for (int k = 0; k < D; k++) {
-            sl.v[k] = (float)((w.c[k] + rimg[r][k]) - wo.c[k]);
-            mloc = fmaxf(mloc, fabsf(sl.v[k]));
+        for (int q = 0; q < kStage; q++) sb[q] = s0 + q * NT;
+        slot_runs<kStage>(runs, sb, rb, jb);
+        typename PosStore<D>::T wb[kStage];
+        float db[kStage];
+        int tb[kStage];
+#pragma unroll
+        for (int q = 0; q < kStage; q++) {
+            if (sb[q] < S) {
+                wb[q] = xw[jb[q]];
+                if constexpr (!UNI) db[q] = (float)diam[jb[q]];
+                if (OUT32) tb[q] = type[jb[q]];
+            }
         }
-        if constexpr (!UNI) sl.v[D] = (float)diam[j];
-        st_f[s] = sl;
-        if (OUT32) st_e[s] = ((uint32_t)j << type_bits) | (uint32_t)type[j];
+#pragma unroll
+        for (int q = 0; q < kStage; q++) {
+            const int s = sb[q];
+            if (s >= S) break;
+            const Vec<D> w = PosStore<D>::load(wb[q]);
+            Slot sl;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                sl.v[k] = (float)((w.c[k] + rimg[rb[q]][k]) - wo.c[k]);
+                mloc = fmaxf(mloc, fabsf(sl.v[k]));
+            }
+            if constexpr (!UNI) sl.v[D] = db[q];
+            st_f[s] = sl;
+            if (OUT32) st_e[s] = ((uint32_t)jb[q] << type_bits) | (uint32_t)tb[q];
+        }
     }
     if (mloc > 0.0f) atomicMax(&mbits, __float_as_int(mloc));  // >= 0: int order
     __syncthreads();
@@ -780,6 +802,15 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                 self = L.off[cc] + (i - L.g0[cc]);
             }
             int cnt = 0;  // the atom's entries so far (same in all its lanes)
+            // off24 (the list spans under 4 GiB and an entry row of npad
+            // atoms under 2^24 entries): entry q's element index in 32-bit
+            // arithmetic, lane row (q >> ts) or row pair (q >> (ts + 1)) times
+            // the row's entries by a 24-bit multiply, plus this atom's and
+            // the lane's place in it (the 2-byte layout packs two rows per
+            // 32-bit word: tiles.hpp tiled_entry)
+            const int ts = tpa_shift(tpa);
+            const uint32_t rowe = OUT32 ? ((uint32_t)npad << ts) : ((uint32_t)npad << (ts + 1));
+            const uint32_t ie = OUT32 ? ((uint32_t)i << ts) : ((uint32_t)i << (ts + 1));
             // The candidate scan, compiled with and without FOLD (a minimum
             // image taken explicitly along some dimension).  With UNI the
             // one cutoff is the same FP32 value the per-pair formula gives.
@@ -788,106 +819,124 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
             const float c2u = cu * cu;
             auto scan = [&](auto FOLDC) {
                 constexpr bool FOLD = decltype(FOLDC)::value;
-                for (int kr = 0; kr < 2 * L.NR; kr++) {
-                    int lo, hi, lo1, hi1;
-                    stencil_ranges(L, g, kr >> 1, cx, lo, hi, lo1, hi1);
-                    if (kr & 1) {
-                        lo = lo1;
-                        hi = hi1;
-                    }
-                    for (int base = lo; base < hi; base += G * U) {
-                        Slot pw[U];
+                for (int k = 0; k < L.NR; k++) {
+                    int lo0, hi0, lo1, hi1;
+                    stencil_ranges(L, g, k, cx, lo0, hi0, lo1, hi1);
+                    // the second range (an end cell of a full-row stencil)
+                    // only when some lane of the wavefront has one
+                    const int nh = __ballot(hi1 > lo1) != 0ull ? 2 : 1;
+                    for (int h = 0; h < nh; h++) {
+                        const int lo = h ? lo1 : lo0, hi = h ? hi1 : hi0;
+                        for (int base = lo; base < hi; base += G * U) {
+                            Slot pw[U];
 #pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            const int s = base + u * G + sub;
-                            pw[u] = st_f[s];
-                        }
-                        // candidate u of this lane is slot base + u G + sub;
-                        // `valid`: the ones below hi that are not the atom
-                        // itself (one mask per round, not a test per slot)
-                        const int first = base + sub;
-                        const int nv = min(U, max(0, (hi - first + G - 1) / G));
-                        uint32_t valid = nv >= 32 ? ~0u : ((1u << nv) - 1u);
-                        {
-                            const int ds = self - first;
-                            if (ds >= 0 && (ds % G) == 0 && ds / G < U)
-                                valid &= ~(1u << (ds / G));
-                        }
-                        // bit u: d2 below the cutoff by more than the
-                        // tolerance (pbits), within it of the cutoff or below
-                        // (ubits); filled from the top so that bit u is
-                        // candidate u
-                        uint32_t pbits = 0, ubits = 0;
-#pragma unroll
-                        for (int u = U - 1; u >= 0; u--) {
-                            float d2 = 0.0f;
-#pragma unroll
-                            for (int k2 = 0; k2 < D; k2++) {
-                                float dk = ri[k2] - pw[u].v[k2];
-                                if (FOLD && foldk[k2]) dk -= Lf[k2] * rintf(dk * iLf[k2]);
-                                d2 = fmaf(dk, dk, d2);
-                            }
-                            float c2 = c2u;
-                            if constexpr (!UNI) {
-                                const float cc = fmaf(0.5f, dif + pw[u].v[UNI ? 0 : D], skf);
-                                c2 = cc * cc;
-                            }
-                            pbits = 2u * pbits + (d2 < c2 - tol ? 1u : 0u);
-                            ubits = 2u * ubits + (d2 <= c2 + tol ? 1u : 0u);
-                        }
-                        pbits &= valid;
-                        ubits &= valid & ~pbits;
-                        // candidates within the tolerance of the cutoff: the
-                        // reference's exact predicate (one wave-uniform branch)
-                        if (__ballot(ubits != 0u)) {
-                            while (ubits) {
-                                const int u = __builtin_ctz(ubits);
-                                ubits &= ubits - 1u;
-                                const int s = first + u * G;
-                                const int cj = slot_cell(L, s);
-                                const int j = L.g0[cj] + (s - L.off[cj]);
-                                if (in_list<D>(xi, PosStore<D>::load(pos[j]), di, diam[j], box,
-                                               skin))
-                                    pbits |= 1u << u;
-                            }
-                        }
-                        // ignored pairs never enter the list (src/trackers.cpp:64)
-                        if (ex.any) {
-                            uint32_t b = pbits;
-                            while (b) {
-                                const int u = __builtin_ctz(b);
-                                b &= b - 1u;
-                                const int s = first + u * G;
-                                const int cj = slot_cell(L, s);
-                                const int j = L.g0[cj] + (s - L.off[cj]);
-                                if (excluded(ex, oi, ex.orig[j])) pbits &= ~(1u << u);
-                            }
-                        }
-                        uint32_t bits = pbits;
-                        const int np = __builtin_popcount(bits);
-                        // exclusive prefix of the round's passes over the group
-                        // (the G lanes of one atom), and the group total
-                        int incl, tot;
-                        group_scan<G>(np, sub, glast, incl, tot);
-                        int q = cnt + incl - np;
-                        cnt += tot;
-                        // the lane's passes in slot order, one store each: the
-                        // wavefront runs as many store rounds as its busiest
-                        // lane has passes (a few), not one per candidate
-                        while (bits) {
-                            const int u = __builtin_ctz(bits);
-                            bits &= bits - 1u;
-                            if (q < cap) {
+                            for (int u = 0; u < U; u++) {
                                 const int s = base + u * G + sub;
-                                if (OUT32)
-                                    static_cast<uint32_t *>(
-                                        nbr)[untiled_entry(q, i, tpa, npad)] =
-                                        st_e[s];
-                                else
-                                    static_cast<uint16_t *>(nbr)[tiled_entry(q, i, tpa, npad)] =
-                                        (uint16_t)s;
+                                pw[u] = st_f[s];
                             }
-                            q++;
+                            // candidate u of this lane is slot base + u G + sub;
+                            // `valid`: the ones below hi that are not the atom
+                            // itself (one mask per round, not a test per slot)
+                            const int first = base + sub;
+                            const int nv = min(U, max(0, (hi - first + G - 1) / G));
+                            uint32_t valid = nv >= 32 ? ~0u : ((1u << nv) - 1u);
+                            {
+                                const int ds = self - first;
+                                if (ds >= 0 && (ds % G) == 0 && ds / G < U)
+                                    valid &= ~(1u << (ds / G));
+                            }
+                            // bit u: d2 below the cutoff by more than the
+                            // tolerance (pbits), within it of the cutoff or below
+                            // (ubits); filled from the top so that bit u is
+                            // candidate u
+                            uint32_t pbits = 0, ubits = 0;
+#pragma unroll
+                            for (int u = U - 1; u >= 0; u--) {
+                                float d2 = 0.0f;
+#pragma unroll
+                                for (int k2 = 0; k2 < D; k2++) {
+                                    float dk = ri[k2] - pw[u].v[k2];
+                                    if (FOLD && foldk[k2]) dk -= Lf[k2] * rintf(dk * iLf[k2]);
+                                    d2 = fmaf(dk, dk, d2);
+                                }
+                                float c2 = c2u;
+                                if constexpr (!UNI) {
+                                    const float cc = fmaf(0.5f, dif + pw[u].v[UNI ? 0 : D], skf);
+                                    c2 = cc * cc;
+                                }
+                                pbits = 2u * pbits + (d2 < c2 - tol ? 1u : 0u);
+                                ubits = 2u * ubits + (d2 <= c2 + tol ? 1u : 0u);
+                            }
+                            pbits &= valid;
+                            ubits &= valid & ~pbits;
+                            // candidates within the tolerance of the cutoff: the
+                            // reference's exact predicate (one wave-uniform branch)
+                            if (__ballot(ubits != 0u)) {
+                                while (ubits) {
+                                    const int u = __builtin_ctz(ubits);
+                                    ubits &= ubits - 1u;
+                                    const int s = first + u * G;
+                                    const int cj = slot_cell(L, s);
+                                    const int j = L.g0[cj] + (s - L.off[cj]);
+                                    if (in_list<D>(xi, PosStore<D>::load(pos[j]), di, diam[j], box,
+                                                   skin))
+                                        pbits |= 1u << u;
+                                }
+                            }
+                            // ignored pairs never enter the list (src/trackers.cpp:64)
+                            if (ex.any) {
+                                uint32_t b = pbits;
+                                while (b) {
+                                    const int u = __builtin_ctz(b);
+                                    b &= b - 1u;
+                                    const int s = first + u * G;
+                                    const int cj = slot_cell(L, s);
+                                    const int j = L.g0[cj] + (s - L.off[cj]);
+                                    if (excluded(ex, oi, ex.orig[j])) pbits &= ~(1u << u);
+                                }
+                            }
+                            uint32_t bits = pbits;
+                            const int np = __builtin_popcount(bits);
+                            // exclusive prefix of the round's passes over the group
+                            // (the G lanes of one atom), and the group total
+                            int incl, tot;
+                            group_scan<G>(np, sub, glast, incl, tot);
+                            int q = cnt + incl - np;
+                            cnt += tot;
+                            // the lane's passes in slot order, one store each: the
+                            // wavefront runs as many store rounds as its busiest
+                            // lane has passes (a few), not one per candidate
+                            while (bits) {
+                                const int u = __builtin_ctz(bits);
+                                bits &= bits - 1u;
+                                if (q < cap) {
+                                    const int s = first + u * G;
+                                    if (off24) {
+                                        const uint32_t qq = (uint32_t)q;
+                                        uint32_t e;
+                                        if (OUT32)
+                                            e = __umul24(qq >> ts, rowe) + ie +
+                                                (qq & (uint32_t)(tpa - 1));
+                                        else
+                                            e = __umul24(qq >> (ts + 1), rowe) + ie +
+                                                ((qq & (uint32_t)(tpa - 1)) << 1) +
+                                                ((qq >> ts) & 1u);
+                                        if (OUT32)
+                                            *reinterpret_cast<uint32_t *>(
+                                                static_cast<char *>(nbr) + (e << 2)) = st_e[s];
+                                        else
+                                            *reinterpret_cast<uint16_t *>(
+                                                static_cast<char *>(nbr) + (e << 1)) = (uint16_t)s;
+                                    } else if (OUT32) {
+                                        static_cast<uint32_t *>(
+                                            nbr)[untiled_entry(q, i, tpa, npad)] = st_e[s];
+                                    } else {
+                                        static_cast<uint16_t *>(nbr)[tiled_entry(q, i, tpa, npad)] =
+                                            (uint16_t)s;
+                                    }
+                                }
+                                q++;
+                            }
                         }
                     }
                 }
@@ -1179,12 +1228,18 @@ hipError_t launch_build_tiled(int ndim, int n, int ntiles, int smax, const TileR
     const size_t lds = (size_t)(smax + kBuildPad) * slot +
                        (out32 ? (size_t)smax * sizeof(uint32_t) : 0);
     const bool sparse = 3.0 * (double)n / (double)g.ncell <= 12.0;  // atoms per stencil row
+    // the build's 32-bit entry addressing (see k_build_tiled): the list under
+    // 4 GiB and one entry row of npad atoms under 2^24 entries
+    const int ts = 31 - __builtin_clz((unsigned)std::max(tpa, 1));
+    const size_t esize = out32 ? 4 : 2;
+    const size_t row = out32 ? ((size_t)npad << ts) : ((size_t)npad << (ts + 1));
+    const int off24 = (size_t)cap * (size_t)npad * esize <= 0xffffffffull && row < (1u << 24);
 #define PARM_BT(DD, PT, O32, UU, UNR)                                                        \
     hipLaunchKernelGGL((k_build_tiled<DD, kTiledLanes, UNR, O32, UU>), dim3(ntiles),          \
                        dim3(kWave * kTiledLanes), lds, s, tiles, truns, smax, (const PT *)pos, \
                        (const PT *)xw, rank, diam, type, type_bits, keys, cstart, g, box,     \
-                       skin, dmax, tpa, cap, npad, nbr, ncount, lcount, (PT *)xref, part,     \
-                       flag, spec, ex)
+                       skin, dmax, tpa, cap, npad, off24, nbr, ncount, lcount, (PT *)xref,    \
+                       part, flag, spec, ex)
 #define PARM_BT_S(DD, PT, O32, UU)                       \
     if (sparse) {                                       \
         PARM_BT(DD, PT, O32, UU, kTiledUnrollSparse);   \

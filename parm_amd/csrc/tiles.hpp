@@ -266,10 +266,12 @@ __device__ __forceinline__ int2 run_entry(const int2 *truns, int t, int r) {
 
 // Atoms of K staged slots s[q] (0 <= s[q] < S): s plus the offset of the last
 // run whose first slot is <= s (empty runs share their first slot with the
-// next run, which the lifting then takes).  The K liftings advance together,
-// one LDS read each per step, so their reads are in flight at once.
+// next run, which the lifting then takes), and that run's index.  The K
+// liftings advance together, one LDS read each per step, so their reads are
+// in flight at once.
 template <int K>
-__device__ __forceinline__ void slot_atoms(const int2 *rs, const int (&s)[K], int (&j)[K]) {
+__device__ __forceinline__ void slot_runs(const int2 *rs, const int (&s)[K], int (&r)[K],
+                                          int (&j)[K]) {
     int lo[K], d[K];
     const int d0 = rs[0].x;
 #pragma unroll
@@ -290,7 +292,16 @@ __device__ __forceinline__ void slot_atoms(const int2 *rs, const int (&s)[K], in
         }
     }
 #pragma unroll
-    for (int q = 0; q < K; q++) j[q] = s[q] + d[q];
+    for (int q = 0; q < K; q++) {
+        j[q] = s[q] + d[q];
+        r[q] = lo[q];
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void slot_atoms(const int2 *rs, const int (&s)[K], int (&j)[K]) {
+    int r[K];
+    slot_runs<K>(rs, s, r, j);
 }
 
 __device__ __forceinline__ int slot_atom(const int2 *rs, int s) {
