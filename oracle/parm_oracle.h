@@ -40,11 +40,11 @@
  * What pins each part (tests/test_oracle.py, tests/test_oracle_kinds.py):
  * exact rationals, closed forms, 50-digit gradients, brute-force lists and
  * the reference's own test assertions; for CollectionNLCG, equilibrium at
- * the goal pressure, the exact reset/descend steps and finite termination
- * of conjugate gradients on an exact quadratic.  Parity unpinned: the
- * Polak-Ribiere numerator's -f_new.f_old term of CollectionNLCG's beta
- * (it vanishes under exact line searches on a quadratic, and the
- * convergence tests would pass without it).
+ * the goal pressure, the exact reset/descend steps, finite termination
+ * of conjugate gradients on an exact quadratic, and the second step's beta
+ * against Polak and Ribiere's coefficient from the reported forces on a
+ * non-quadratic packing (where Fletcher and Reeves' coefficient, the same
+ * without the -f_new.f_old term, differs by over 20 %).
  */
 #ifndef PARM_ORACLE_H
 #define PARM_ORACLE_H

@@ -305,3 +305,31 @@ def test_nlcg_conjugate_gradient_terminates_on_a_quadratic(seed):
     s2, c2 = _anchored_chain(seed, kmax=1)
     c2.timestep(4)
     assert fmax(s2) > 1e-6 * f0
+
+@pytest.mark.parametrize("seed", [0, 1, 3])
+def test_nlcg_beta_is_polak_ribiere(seed):
+    """The second CollectionNLCG step's beta is Polak and Ribiere's
+    coefficient (the reference names the method at src/collection.cpp:696
+    and forms it at :802-815): beta = f2.(f2 - f1) / f1.f1, with f1 the forces
+    after the first step and f2 after the second, in the step's inner
+    product (sum over mobile atoms; the box terms vanish with kappa = 1e30,
+    which also freezes the box, so the length-squared normaliser cancels).
+    After the first step f2.f1 is not zero here (harmonic repulsion is not
+    quadratic in the positions), so Fletcher and Reeves' f2.f2 / f1.f1, the
+    same formula without the -f2.f1 term, differs by over 20 %.  The clamp to
+    [0, 1] gives betaused."""
+    s = _packing(30, 3, 0.6, seed=seed)
+    c = O.NLCG(s, 1e-3, 0.0, kappa=1e30)
+    c.set_limits(alphamax=0.0, afrac=0.0, dxmax=0.0, stepmax=0.0, maxdV=0.0)
+    c.reset()
+    c.timestep(1)
+    f1 = s.state()[2].copy()  # a = f after a step
+    c.timestep(1)
+    f2 = s.state()[2].copy()
+    info = c.info()
+    beta_pr = (np.sum(f2 * f2) - np.sum(f2 * f1)) / np.sum(f1 * f1)
+    beta_fr = np.sum(f2 * f2) / np.sum(f1 * f1)
+    assert abs(beta_fr - beta_pr) > 0.2 * abs(beta_pr)
+    assert info["beta"] == pytest.approx(beta_pr, rel=1e-12)
+    assert info["betaused"] == min(max(beta_pr, 0.0), 1.0) or \
+        info["betaused"] == pytest.approx(min(max(beta_pr, 0.0), 1.0), rel=1e-12)
