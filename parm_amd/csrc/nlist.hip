@@ -817,6 +817,7 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
             const bool anyfold = foldk[0] || (D >= 2 && foldk[1]) || (D == 3 && foldk[2]);
             const float cu = fmaf(0.5f, dif + dif, skf);
             const float c2u = cu * cu;
+            const float c2lou = c2u - tol, c2hiu = c2u + tol;
             auto scan = [&](auto FOLDC) {
                 constexpr bool FOLD = decltype(FOLDC)::value;
                 for (int k = 0; k < L.NR; k++) {
@@ -846,9 +847,14 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                                     valid &= ~(1u << (ds / G));
                             }
                             // bit u: d2 below the cutoff by more than the
-                            // tolerance (pbits), within it of the cutoff or below
-                            // (ubits); filled from the top so that bit u is
-                            // candidate u
+                            // tolerance (pbits), below the cutoff plus the
+                            // tolerance (ubits); filled from the top so that bit
+                            // u is candidate u.  Each bit is the sign of d2 minus
+                            // the bound (exact in sign: the difference of two
+                            // finite floats rounds to zero only when they are
+                            // equal), shifted in by one funnel shift.  At
+                            // d2 = c2 + tol the candidate counts as out: the
+                            // tolerance is four times the FP32 error bound.
                             uint32_t pbits = 0, ubits = 0;
 #pragma unroll
                             for (int u = U - 1; u >= 0; u--) {
@@ -859,13 +865,15 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                                     if (FOLD && foldk[k2]) dk -= Lf[k2] * rintf(dk * iLf[k2]);
                                     d2 = fmaf(dk, dk, d2);
                                 }
-                                float c2 = c2u;
+                                float c2lo = c2lou, c2hi = c2hiu;
                                 if constexpr (!UNI) {
                                     const float cc = fmaf(0.5f, dif + pw[u].v[UNI ? 0 : D], skf);
-                                    c2 = cc * cc;
+                                    const float c2 = cc * cc;
+                                    c2lo = c2 - tol;
+                                    c2hi = c2 + tol;
                                 }
-                                pbits = 2u * pbits + (d2 < c2 - tol ? 1u : 0u);
-                                ubits = 2u * ubits + (d2 <= c2 + tol ? 1u : 0u);
+                                pbits = __builtin_amdgcn_alignbit(pbits, __float_as_uint(d2 - c2lo), 31u);
+                                ubits = __builtin_amdgcn_alignbit(ubits, __float_as_uint(d2 - c2hi), 31u);
                             }
                             pbits &= valid;
                             ubits &= valid & ~pbits;
