@@ -641,7 +641,7 @@ __device__ __forceinline__ void group_scan(int v, int sub, int glast, int &incl,
 // layout) instead of staged slots, for systems whose force kernel runs on the
 // untiled list.  UNI: every member has the same list diameter; the staged
 // slot then holds the offsets only (one cutoff for every candidate).
-constexpr int kBuildPad = 32;  // staged slots of padding after the stencil
+constexpr int kBuildPad = 64;  // staged slots of padding after the stencil
 
 template <int D, bool UNI>
 struct BuildSlot {
@@ -668,6 +668,9 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
     Slot *st_f = reinterpret_cast<Slot *>(st_raw);
     uint32_t *st_e = reinterpret_cast<uint32_t *>(st_f + smax + kBuildPad);  // OUT32 entries
     static_assert(G * U <= kBuildPad, "a round stays inside the padded stage");
+    static_assert(U <= 32, "a round's candidates are the bits of one word");
+    constexpr int kBatch = U < 4 ? U : 4;  // slots read and tested together
+    static_assert(U % kBatch == 0, "a round is whole batches");
     __shared__ TileLayout L;
     __shared__ int2 runs[kRunSlots];  // run_entry form (tiles.hpp)
     __shared__ double rimg[kTileRuns][3];
@@ -775,13 +778,15 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
             // along x the minimum image is explicit for a collapsed
             // dimension or a full-row stencil (no staged shift)
             const bool fold0 = g.nc[0] < 3 || L.xfull;
+            // Lf: the box edge along the folded dimensions, 0 along the
+            // others (their fold then subtracts nothing: branch-free)
             bool foldk[3];
             float Lf[3], iLf[3], ri[3];
             double Lfold = 0.0;
 #pragma unroll
             for (int k = 0; k < D; k++) {
                 foldk[k] = k == 0 ? fold0 : g.nc[k] < 3;
-                Lf[k] = (float)box.L[k];
+                Lf[k] = foldk[k] ? (float)box.L[k] : 0.0f;
                 iLf[k] = (float)box.invL[k];
                 ri[k] = (float)(wi.c[k] - wo.c[k]);
                 if (foldk[k]) Lfold = fmax(Lfold, box.L[k]);
@@ -829,12 +834,6 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                     for (int h = 0; h < nh; h++) {
                         const int lo = h ? lo1 : lo0, hi = h ? hi1 : hi0;
                         for (int base = lo; base < hi; base += G * U) {
-                            Slot pw[U];
-#pragma unroll
-                            for (int u = 0; u < U; u++) {
-                                const int s = base + u * G + sub;
-                                pw[u] = st_f[s];
-                            }
                             // candidate u of this lane is slot base + u G + sub;
                             // `valid`: the ones below hi that are not the atom
                             // itself (one mask per round, not a test per slot)
@@ -855,25 +854,40 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                             // equal), shifted in by one funnel shift.  At
                             // d2 = c2 + tol the candidate counts as out: the
                             // tolerance is four times the FP32 error bound.
+                            // The slots are read and tested kBatch at a time (the
+                            // last batch first), the scheduler kept from hoisting
+                            // the next batch's reads above this one's tests: the
+                            // round's registers stay those of one batch.
                             uint32_t pbits = 0, ubits = 0;
 #pragma unroll
-                            for (int u = U - 1; u >= 0; u--) {
-                                float d2 = 0.0f;
+                            for (int b = U / kBatch - 1; b >= 0; b--) {
+                                Slot pw[kBatch];
 #pragma unroll
-                                for (int k2 = 0; k2 < D; k2++) {
-                                    float dk = ri[k2] - pw[u].v[k2];
-                                    if (FOLD && foldk[k2]) dk -= Lf[k2] * rintf(dk * iLf[k2]);
-                                    d2 = fmaf(dk, dk, d2);
+                                for (int u = 0; u < kBatch; u++)
+                                    pw[u] = st_f[first + (b * kBatch + u) * G];
+#pragma unroll
+                                for (int u = kBatch - 1; u >= 0; u--) {
+                                    float d2 = 0.0f;
+#pragma unroll
+                                    for (int k2 = 0; k2 < D; k2++) {
+                                        float dk = ri[k2] - pw[u].v[k2];
+                                        if (FOLD) dk = fmaf(-Lf[k2], rintf(dk * iLf[k2]), dk);
+                                        d2 = fmaf(dk, dk, d2);
+                                    }
+                                    float c2lo = c2lou, c2hi = c2hiu;
+                                    if constexpr (!UNI) {
+                                        const float cc =
+                                            fmaf(0.5f, dif + pw[u].v[UNI ? 0 : D], skf);
+                                        const float c2 = cc * cc;
+                                        c2lo = c2 - tol;
+                                        c2hi = c2 + tol;
+                                    }
+                                    pbits = __builtin_amdgcn_alignbit(
+                                        pbits, __float_as_uint(d2 - c2lo), 31u);
+                                    ubits = __builtin_amdgcn_alignbit(
+                                        ubits, __float_as_uint(d2 - c2hi), 31u);
                                 }
-                                float c2lo = c2lou, c2hi = c2hiu;
-                                if constexpr (!UNI) {
-                                    const float cc = fmaf(0.5f, dif + pw[u].v[UNI ? 0 : D], skf);
-                                    const float c2 = cc * cc;
-                                    c2lo = c2 - tol;
-                                    c2hi = c2 + tol;
-                                }
-                                pbits = __builtin_amdgcn_alignbit(pbits, __float_as_uint(d2 - c2lo), 31u);
-                                ubits = __builtin_amdgcn_alignbit(ubits, __float_as_uint(d2 - c2hi), 31u);
+                                if (b > 0) __asm__ volatile("" ::: "memory");
                             }
                             pbits &= valid;
                             ubits &= valid & ~pbits;
@@ -1213,9 +1227,11 @@ hipError_t launch_tiles(int ndim, int n, const CellGrid &g, const uint32_t *keys
 
 constexpr int kTiledLanes = 4;   // lanes per atom in k_build_tiled
 // candidates per lane per round: a stencil row (3 cells) of a dense system
-// (the LJ liquid: ~50 atoms) fills two rounds of 4 x 8; a sparse one (packed
-// discs / spheres: ~7-9 atoms) one round of 4 x 2
-constexpr int kTiledUnroll = 8, kTiledUnrollSparse = 2;
+// (the LJ liquid: ~50 atoms) fits one round of 4 x 16 (one prefix scan and
+// one store loop for the row; 4 x 8 took two of each and was 7 % slower at
+// 1M atoms); a sparse one (packed discs / spheres: ~7-9 atoms) one round of
+// 4 x 2
+constexpr int kTiledUnroll = 16, kTiledUnrollSparse = 2;
 
 size_t tiled_build_lds(int smax) {
     return (size_t)(smax + kBuildPad) * sizeof(float4) + (size_t)smax * sizeof(uint32_t);
