@@ -925,13 +925,25 @@ __global__ __launch_bounds__(kWave *G) void k_build_tiled(
                             group_scan<G>(np, sub, glast, incl, tot);
                             int q = cnt + incl - np;
                             cnt += tot;
+                            // entries at or past the capacity are not stored (the
+                            // count is reported and the build repeated with more):
+                            // only the lane's lowest cap - q passes stay
+                            if (q + np > cap) {
+                                uint32_t b = bits;
+                                bits = 0u;
+                                for (int k = max(cap - q, 0); k > 0 && b; k--) {
+                                    const uint32_t low = b & (0u - b);
+                                    bits |= low;
+                                    b ^= low;
+                                }
+                            }
                             // the lane's passes in slot order, one store each: the
                             // wavefront runs as many store rounds as its busiest
                             // lane has passes (a few), not one per candidate
                             while (bits) {
                                 const int u = __builtin_ctz(bits);
                                 bits &= bits - 1u;
-                                if (q < cap) {
+                                {
                                     const int s = first + u * G;
                                     if (off24) {
                                         const uint32_t qq = (uint32_t)q;

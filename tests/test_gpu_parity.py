@@ -417,6 +417,36 @@ def test_rebuild_after_denser_packing_replans():
         S.assert_fp64_forces(s.forces(), ref, xs, L, t, first, last)
 
 
+def test_list_capacity_overflow_regrows():
+    """Half the atoms squeezed into a quarter of the box (twice the density
+    there): the fullest lists exceed the per-atom capacity the first build
+    takes from the mean density (1.35 x the mean count + 24), so that build
+    stores only the entries below the capacity, reports the counts, and is
+    repeated with a larger capacity.  Pair set and FP64 forces match the
+    oracle."""
+    _device_or_skip()
+    n = 32768
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=5)
+    p = S.lj_params(n)
+    t = _table(O.LJ_ATTRACT_REPULSE, p)
+    xs = x - L * np.floor(x / L + 0.5)  # into [-L/2, L/2)
+    left = xs[:, 0] < 0.0
+    xs[left, 0] *= 0.5
+    first, last = O.nlist(xs, L, p["diam"], 0.3)
+    counts = np.bincount(np.concatenate([first, last]), minlength=n)
+    r = p["diam"].max() + 0.3
+    cap0 = 1.35 * n / np.prod(L) * (4.0 / 3.0 * np.pi * r**3) + 24.0
+    assert counts.max() > cap0 + 10
+    print(f"capacity overflow: max count {counts.max()}, first capacity {cap0:.0f}", flush=True)
+    s = _make(O.LJ_ATTRACT_REPULSE, xs, L, p, 0.3)
+    assert s.update_list(True)
+    f_dev, l_dev = s.pairs()
+    assert np.array_equal(f_dev, first) and np.array_equal(l_dev, last)
+    s.set_forces(False)
+    ref = O.forces(xs, L, t, first, last)
+    S.assert_fp64_forces(s.forces(), ref, xs, L, t, first, last)
+
+
 @pytest.mark.parametrize("case", ["empty", "no_members", "single"])
 def test_empty_and_memberless_systems(case):
     """An AtomVec with no atoms, atoms none of which is a list member, and a
