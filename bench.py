@@ -19,8 +19,10 @@ the steady-state throughput, and the per-atom force error of the mixed form.
 
 Roofline: the fused step kernel (forces + kick + next drift + skin test) is
 timed with HIP events on its own stream (the end event is recorded right
-after the kernel, before its k_finalize reduction launches); its algorithmic
-bytes per launch are
+after the kernel, before its k_finalize reduction launches), every
+PROFILE_EVERY-th launch of the timed steps (an event pair costs the stream a
+few microseconds, ~3 % of a step when every launch carries one); its
+algorithmic bytes per launch are
 N * (192 + 4 * nbar) with nbar the mean full-list length (FP64 state read and
 written once, one int32 list index per entry, neighbour gathers counted as
 cache hits).  Peak: 8 TB/s HBM3E.
@@ -86,6 +88,9 @@ def total_energy(s):
     return s.pair_sums()["U"] + s.kinetic_energy()
 
 
+PROFILE_EVERY = 4
+
+
 def run_timed(s, dt, steps, warmup):
     """Autotunes the step kernel (timed on its own), runs `warmup` untimed
     steps, then times exactly `steps` steps bracketed by device
@@ -99,7 +104,7 @@ def run_timed(s, dt, steps, warmup):
         s.verlet(dt, warmup)
     s.synchronize()
     s.profile_read(reset=True)
-    s.profile_enable(True)
+    s.profile_enable(True, every=PROFILE_EVERY)
     t0 = time.perf_counter()
     s.verlet(dt, steps)
     s.synchronize()

@@ -323,9 +323,11 @@ int parm_snapshot_write_xyz(parm_system *sys, int64_t ticket, const char *path, 
 
 /* ------------------------------------------------------ instrumentation */
 /* No reference counterpart (the reference has no timers); for benchmarks.
- * While enabled, every fused step-kernel launch (force + kick + next drift +
- * skin test) and every list rebuild issued by parm_verlet_steps /
- * parm_damped_steps is bracketed by HIP events on the system's stream.
+ * enable = k >= 1: every k-th fused step-kernel launch (force + kick + next
+ * drift + skin test; k = 1 every one) and every list rebuild issued by
+ * parm_verlet_steps / parm_damped_steps is bracketed by HIP events on the
+ * system's stream (an event pair costs the stream a few microseconds);
+ * enable = 0 turns it off.
  * parm_profile_read waits for the stream and returns the summed durations in
  * milliseconds and the counts since the last reset (reset != 0 clears them). */
 int parm_profile_enable(parm_system *sys, int enable);

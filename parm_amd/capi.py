@@ -379,8 +379,9 @@ class System:
     def synchronize(self):
         _check(lib().parm_synchronize(self.h))
 
-    def profile_enable(self, on=True):
-        _check(lib().parm_profile_enable(self.h, int(bool(on))))
+    def profile_enable(self, on=True, every=1):
+        """Time every `every`-th fused step launch and every rebuild."""
+        _check(lib().parm_profile_enable(self.h, int(every) if on else 0))
 
     def profile_read(self, reset=True):
         sm, rm = C.c_double(0), C.c_double(0)

@@ -180,8 +180,12 @@ struct parm_system {
     hipStream_t out_stream = nullptr;
     int64_t snap_next = 0;
 
-    // instrumentation: event pairs around step-kernel launches and rebuilds
+    // instrumentation: event pairs around step-kernel launches (every
+    // prof_every-th one: an event pair costs the stream a few microseconds)
+    // and rebuilds
     bool prof = false;
+    int prof_every = 1;
+    int64_t prof_seq = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_step, prof_rebuild;
@@ -668,9 +672,10 @@ struct ProfScope {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> *dst;
     hipEvent_t a = nullptr, b = nullptr;
     bool callee_end = false;
-    ProfScope(parm_system *s_, std::vector<std::pair<hipEvent_t, hipEvent_t>> *d)
+    ProfScope(parm_system *s_, std::vector<std::pair<hipEvent_t, hipEvent_t>> *d,
+              bool sample = true)
         : s(s_), dst(d) {
-        if (!s->prof) return;
+        if (!s->prof || !sample) return;
         a = pool_event(s);
         b = pool_event(s);
         if (a && b) (void)hipEventRecord(a, s->stream);
@@ -821,7 +826,7 @@ int run_steps(parm_system *s, bool damped, const StepCoef &c, int64_t nsteps) {
         const int slot = (int)(q % kFlagRing), next = (int)((q + 1) % kFlagRing);
         if (!last) {
             {
-                ProfScope ps(s, &s->prof_step);
+                ProfScope ps(s, &s->prof_step, s->prof_seq++ % s->prof_every == 0);
                 ReduceBuf R = s->redbuf(next);
                 R.done = ps.end();
                 PARM_HIP(launch_force(s->ndim, s->prec, kick_drift, false,
@@ -1657,7 +1662,10 @@ int parm_autotune(parm_system *s, double dt, int damped) {
 
 int parm_profile_enable(parm_system *s, int enable) {
     PARM_CHECK(check_sys(s));
+    if (enable < 0) return fail(PARM_E_INVALID, "profile stride must be >= 0");
     s->prof = enable != 0;
+    s->prof_every = std::max(enable, 1);
+    s->prof_seq = 0;
     return PARM_OK;
 }
 

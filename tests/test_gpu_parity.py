@@ -505,3 +505,37 @@ def test_nonfinite_position_is_reported(integrator):
         else:
             s.damped(0.002, 0.5, 30)
     assert e.value.code == capi.E_NONFINITE, str(e.value)
+
+
+@pytest.mark.parametrize("every", [1, 4])
+def test_profile_samples_every_kth_step_launch(every, monkeypatch):
+    """Instrumentation: with stride k the fused step launches of a run carry
+    timing events at launches 0, k, 2k, ... (a 20-step run makes 19 fused
+    launches; the last step is a plain force + kick), every rebuild carries
+    them, and the trajectory is the same with and without them (the list
+    layout pinned, so the autotuner cannot pick differently)."""
+    _device_or_skip()
+    monkeypatch.setenv("PARM_TPA", "4")
+    monkeypatch.setenv("PARM_TILED", "1")
+    n = 4096
+    x, L = S.jittered_lattice(n, 0.8, 3, seed=31)
+    p = S.lj_params(n)
+    v = S.gaussian_velocities(n, 3, 1.0, seed=31)
+    runs = []
+    for prof in (True, False):
+        s = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3)
+        s.set_state(v=v, a=np.zeros_like(x))
+        s.initialize()
+        s.profile_enable(prof, every=every)
+        w0 = s.which()
+        s.verlet(0.002, 20)
+        r = s.profile_read(reset=True)
+        runs.append((s.state()[0], s.which() - w0, r))
+    (x1, nreb, r), (x2, nreb2, r2) = runs
+    assert r["step_launches"] == len(range(0, 19, every))
+    assert r["step_ms"] > 0.0
+    assert r["rebuilds"] == nreb == nreb2
+    assert r2["step_launches"] == 0 and r2["rebuilds"] == 0
+    assert np.array_equal(x1, x2)
+    with pytest.raises(capi.ParmValueError):
+        s.profile_enable(True, every=-1)
