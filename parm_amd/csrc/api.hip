@@ -158,6 +158,8 @@ struct parm_system {
 
     // reductions and flags
     double *part = nullptr, *out = nullptr;
+    uint32_t *fcnt = nullptr;   // the multi-workgroup skin reduction's counter
+    double *fgpart = nullptr;   // and its per-workgroup results
     uint32_t *dflag = nullptr;      // flag words (see kSlot*)
     uint32_t *hflag = nullptr;      // pinned mirror, written by the kernels
     uint32_t *hflag_dev = nullptr;  // its device-side address
@@ -252,6 +254,8 @@ struct parm_system {
         R.flag = dflag + flag_index;
         R.skin = skin;
         R.hflag = hflag_dev + flag_index;
+        R.cnt = fcnt;
+        R.gpart = fgpart;
         return R;
     }
 };
@@ -948,6 +952,8 @@ void destroy(parm_system *s) {
     dfree(s->ex_part);
     dfree(s->part);
     dfree(s->out);
+    dfree(s->fcnt);
+    dfree(s->fgpart);
     dfree(s->dflag);
     if (s->hflag) (void)hipHostFree(s->hflag);
     if (s->hrep) (void)hipHostFree(s->hrep);
@@ -1138,7 +1144,8 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
         (rc = dalloc(&s->stats, 2)) ||
         (rc = dalloc(&s->bpart, 2 * (size_t)build_blocks((int)n))) ||
         (rc = dalloc(&s->part, (size_t)kRedWidth * force_blocks((int)n, 8))) ||
-        (rc = dalloc(&s->out, kRedWidth)) ||
+        (rc = dalloc(&s->out, kRedWidth)) || (rc = dalloc(&s->fcnt, 1)) ||
+        (rc = dalloc(&s->fgpart, 3 * kFinBlocks)) ||
         (rc = dalloc(&s->dflag, 2 * kFlagRing + 4)) || (rc = dalloc(&s->tinfo, 4)))
         return bail(rc);
     if (hipHostMalloc((void **)&s->hflag, (2 * kFlagRing + 4) * sizeof(uint32_t),
@@ -1165,6 +1172,7 @@ int parm_system_create(int ndim, int precision, int64_t natoms, const double *ma
     (void)hipMemsetAsync(s->type, 0, n * sizeof(int32_t), st);
     (void)hipMemsetAsync(s->diam, 0, n * sizeof(double), st);
     (void)hipMemsetAsync(s->dflag, 0, (2 * kFlagRing + 4) * sizeof(uint32_t), st);
+    (void)hipMemsetAsync(s->fcnt, 0, sizeof(uint32_t), st);
     for (int k = 0; k < 2 * kFlagRing + 4; k++) s->hflag[k] = 0;
     hipLaunchKernelGGL(k_iota, dim3(nb), dim3(256), 0, st, (int)n, s->orig);
     hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, (int)n, s->rank, -1);

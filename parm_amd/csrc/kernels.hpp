@@ -197,7 +197,10 @@ struct ReduceBuf {
     uint32_t *hflag;              // device-visible pinned host copy of the flag word, or null
     hipEvent_t done;              // host side, or null: recorded on the launch stream right
                                   // after the force kernel, before its k_finalize (timing)
+    uint32_t *cnt;                // or null: the skin-test reduction over several
+    double *gpart;                // workgroups (counter, kFinBlocks x 3 results)
 };
+constexpr int kFinBlocks = 32;    // workgroups of the multi-workgroup skin reduction
 constexpr int kRedWidth = 16;     // U, W, S[9], contacts, overlaps, top1, top2, nonfinite
 
 hipError_t launch_force(int ndim, int precision, int stage, bool energy,
