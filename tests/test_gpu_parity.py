@@ -539,3 +539,35 @@ def test_profile_samples_every_kth_step_launch(every, monkeypatch):
     assert np.array_equal(x1, x2)
     with pytest.raises(capi.ParmValueError):
         s.profile_enable(True, every=-1)
+
+
+def test_skin_reduction_over_many_workgroups():
+    """The skin test of 2.2 M atoms leaves more than 8192 workgroup partials,
+    which several workgroups reduce with a last-workgroup merge: the two
+    largest displacements, of two atoms far apart in the atom order, decide
+    the rebuild at the reference's threshold (d1 + d2 >= skin,
+    src/trackers.cpp:23-53), and a non-finite position is reported."""
+    _device_or_skip()
+    n = 2_200_000
+    rng = np.random.default_rng(12)
+    L = np.full(3, (n / 0.05) ** (1.0 / 3.0))
+    x = rng.uniform(-L / 2, L / 2, (n, 3))
+    p = S.lj_params(n)
+    s = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3)
+    assert s.update_list(True)
+    w0 = s.which()
+    for d1, d2, expect in ((0.2, 0.0999, False), (0.1999, 0.0, False),
+                           (0.2, 0.1001, True)):
+        print(f"skin case d1={d1} d2={d2}", flush=True)
+        xs = x.copy()
+        xs[0, 0] += d1
+        xs[n - 1, 1] -= d2
+        s.set_state(x=xs)
+        assert s.update_list(False) == expect, (d1, d2)
+        assert s.which() == w0 + (1 if expect else 0)
+    xs = x.copy()
+    xs[n // 2, 2] = np.inf
+    s.set_state(x=xs)
+    with pytest.raises(capi.ParmError) as e:
+        s.update_list(False)
+    assert e.value.code == capi.E_NONFINITE
