@@ -19,7 +19,7 @@ the steady-state throughput, and the per-atom force error of the mixed form.
 
 Roofline: the fused step kernel (forces + kick + next drift + skin test) is
 timed with HIP events on its own stream (the end event is recorded right
-after the kernel, before its k_finalize reduction launches), every
+after the kernel, before its skin-test reduction launches), every
 PROFILE_EVERY-th launch of the timed steps (an event pair costs the stream a
 few microseconds, ~3 % of a step when every launch carries one); its
 algorithmic bytes per launch are
