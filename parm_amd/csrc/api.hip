@@ -312,17 +312,20 @@ void make_grid(parm_system *s) {
 
 // Per-lane entry counts are 16-bit (lcount): a list whose capacity could
 // give one lane more than 65535 entries is refused, not wrapped.
-int check_lane_counts(const parm_system *s) {
-    if ((int64_t)s->cap > (int64_t)65535 * s->tpa)
+int check_lane_counts(int64_t cap, int tpa) {
+    if (cap > (int64_t)65535 * tpa)
         return fail(PARM_E_UNSUPPORTED,
-                    "more than 65535 list entries per lane (capacity " + std::to_string(s->cap) +
-                        " at " + std::to_string(s->tpa) +
+                    "more than 65535 list entries per lane (capacity " + std::to_string(cap) +
+                        " at " + std::to_string(tpa) +
                         " lanes per atom): use more lanes per atom or a smaller skin");
     return PARM_OK;
 }
+int check_lane_counts(const parm_system *s) { return check_lane_counts(s->cap, s->tpa); }
 
 int ensure_list_capacity(parm_system *s, int cap) {
     if (cap <= s->cap && s->nbr) return PARM_OK;
+    // refused before the list is reallocated
+    PARM_CHECK(check_lane_counts(((int64_t)cap + 7) & ~(int64_t)7, s->tpa));
     free_list(s);
     s->cap = (cap + 7) & ~7;
     PARM_CHECK(dalloc(&s->nbr, (size_t)s->cap * s->npad));

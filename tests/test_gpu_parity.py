@@ -571,3 +571,21 @@ def test_skin_reduction_over_many_workgroups():
     with pytest.raises(capi.ParmError) as e:
         s.update_list(False)
     assert e.value.code == capi.E_NONFINITE
+
+
+def test_lane_count_bound_refused_before_allocation(monkeypatch):
+    """Per-lane list counts are 16-bit: 70,000 atoms in a cube of edge 1.6
+    (every pair within the list radius 2.8) at one lane per atom need
+    69,999 entries in a lane; the build that finds it reports
+    PARM_E_UNSUPPORTED instead of growing the list past the bound."""
+    _device_or_skip()
+    monkeypatch.setenv("PARM_TPA", "1")
+    n = 70_000
+    rng = np.random.default_rng(3)
+    L = np.full(3, 50.0)
+    x = rng.uniform(-0.8, 0.8, (n, 3))
+    p = S.lj_params(n)
+    s = _make(O.LJ_ATTRACT_REPULSE, x, L, p, 0.3)
+    with pytest.raises((capi.ParmError, capi.ParmValueError)) as e:
+        s.update_list(True)
+    assert e.value.code == capi.E_UNSUPPORTED, str(e.value)
